@@ -1,0 +1,174 @@
+/*
+ * tasktorrent_amd: C-ABI of an MI355X-native (gfx950) tiled dense Cholesky.
+ *
+ * This is the drop-in boundary for the hot path of the reference's
+ * miniapp/dense_cholesky/2d_cholesky.cpp (tile tasks POTRF/TRSM/SYRK/GEMM
+ * seeded with indegree/fulfill_promise and run on a Threadpool). Every entry
+ * point below names the reference interface it replaces (file:line, paths
+ * relative to the reference repository root).
+ *
+ * Conventions: plain pointers and sizes; matrices column-major; int status
+ * codes (TTAD_OK = 0); ttad_last_error() gives a message for the calling
+ * thread. `stream` is a hipStream_t passed as void* (NULL = the context's own
+ * stream). Device pointers are HBM addresses on the context's device.
+ * One host thread per context at a time.
+ */
+#ifndef TASKTORRENT_AMD_H
+#define TASKTORRENT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+#define TTAD_OK 0
+#define TTAD_ERR_INVALID 1     /* bad argument */
+#define TTAD_ERR_OOM 2         /* device allocation failed or over the HBM budget */
+#define TTAD_ERR_NOT_SPD 3     /* non-positive pivot; *info = 1-based global column */
+#define TTAD_ERR_TIMEOUT 4     /* device watchdog or host wait timed out */
+#define TTAD_ERR_DEVICE 5      /* HIP runtime error */
+#define TTAD_ERR_IR_STALLED 6  /* iterative refinement did not reach the target */
+#define TTAD_ERR_GRAPH 7       /* inconsistent task graph (indegree vs edges) */
+#define TTAD_ERR_BUSY 8        /* resource in use (graph running, serving queue full) */
+#define TTAD_ERR_NOT_FOUND 9   /* unknown handle/id */
+
+/* ---- arithmetic / storage ----------------------------------------------- */
+#define TTAD_F64 0   /* FP64 tiles, FP64 MFMA */
+#define TTAD_F32 1   /* FP32 tiles, FP32 MFMA */
+#define TTAD_BF16 2  /* FP32 tiles, BF16 MFMA inputs with FP32 accumulation */
+
+/* ---- task kinds of the Cholesky PTG (2d_cholesky.cpp:316-608) ------------ */
+#define TTAD_POTRF 0
+#define TTAD_TRSM 1
+#define TTAD_SYRK 2
+#define TTAD_GEMM 3
+
+/* ---- schedules of ttad_potrf_tiled ------------------------------------------ */
+#define TTAD_SCHED_STREAM 0      /* one launch per panel step (bulk-synchronous) */
+#define TTAD_SCHED_PERSISTENT 1  /* device-resident DAG with dependency counters */
+
+typedef struct ttad_context ttad_context;
+typedef struct ttad_matrix ttad_matrix;
+typedef struct ttad_graph ttad_graph;
+
+/* ---- library / context ---------------------------------------------------- */
+const char* ttad_last_error(void);
+int ttad_version(void);
+/* Replaces the Threadpool construction (threadpool_shared.cpp:17-33): binds a
+ * device, creates its stream and scratch. */
+int ttad_init(int device, ttad_context** out);
+int ttad_finalize(ttad_context* ctx);
+int ttad_device_info(ttad_context* ctx, int* n_cu, long long* hbm_bytes, char* name, int name_len);
+int ttad_synchronize(ttad_context* ctx, void* stream);
+
+/* ---- tile kernels: BLAS/LAPACK-shaped, device pointers, column-major ----------- */
+/* LAPACKE_dpotrf(LAPACK_COL_MAJOR, 'L', n, A, lda)  (2d_cholesky.cpp:319).
+ * *info as LAPACK (0 ok, k > 0 non-positive pivot at column k). Synchronous. */
+int ttad_dpotrf(ttad_context* ctx, int n, double* A, int lda, int* info, void* stream);
+/* cblas_dtrsm(ColMajor, Right, Lower, Trans, NonUnit, m, n, 1.0, L, ldl, B, ldb)
+ * (2d_cholesky.cpp:404-405): B := B L^{-T}. */
+int ttad_dtrsm(ttad_context* ctx, int m, int n, const double* L, int ldl, double* B, int ldb, void* stream);
+/* cblas_dsyrk(ColMajor, Lower, NoTrans, n, k, alpha, A, lda, beta, C, ldc)
+ * (2d_cholesky.cpp:490-491): lower triangle of C := beta C + alpha A A^T. */
+int ttad_dsyrk(ttad_context* ctx, int n, int k, double alpha, const double* A, int lda, double beta, double* C,
+               int ldc, void* stream);
+/* cblas_dgemm(ColMajor, NoTrans, Trans, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)
+ * (2d_cholesky.cpp:493-494): C := beta C + alpha A B^T. */
+int ttad_dgemm(ttad_context* ctx, int m, int n, int k, double alpha, const double* A, int lda, const double* B,
+               int ldb, double beta, double* C, int ldc, void* stream);
+/* FP32 versions of the same four calls (mixed-precision variants; no reference
+ * counterpart, the reference is FP64 only). ttad_spotrf factors in FP64. */
+int ttad_spotrf(ttad_context* ctx, int n, float* A, int lda, int* info, void* stream);
+int ttad_strsm(ttad_context* ctx, int m, int n, const float* L, int ldl, float* B, int ldb, void* stream);
+int ttad_ssyrk(ttad_context* ctx, int n, int k, float alpha, const float* A, int lda, float beta, float* C, int ldc,
+               void* stream);
+int ttad_sgemm(ttad_context* ctx, int m, int n, int k, float alpha, const float* A, int lda, const float* B, int ldb,
+               float beta, float* C, int ldc, void* stream);
+/* C := beta C + alpha bf16(A) bf16(B)^T with FP32 storage and accumulation
+ * (BF16 trailing-update variant; syrk != 0 updates the lower triangle only). */
+int ttad_gemm_bf16(ttad_context* ctx, int m, int n, int k, float alpha, const float* A, int lda, const float* B,
+                   int ldb, float beta, float* C, int ldc, int syrk, void* stream);
+
+/* ---- tiled symmetric matrices in HBM ------------------------------------------ */
+/* Replaces the per-tile storage `blocks[ii + jj*num_blocks]` of 2d_cholesky.cpp:134-147
+ * with lda = block size (block_sizes_lda, :86-89). Only tiles i >= j exist.
+ * storage: TTAD_F64 or TTAD_F32. */
+int ttad_matrix_create(ttad_context* ctx, int num_blocks, const int* block_sizes, int storage, ttad_matrix** out);
+int ttad_matrix_destroy(ttad_matrix* M);
+/* Copy the lower triangle of a full n x n column-major FP64 array in (set) or out
+ * (get). *_device != 0: the array is in HBM, else in host memory. get writes the
+ * strict upper triangle as zeros. */
+int ttad_matrix_set(ttad_matrix* M, const double* A, long long lda, int src_device, void* stream);
+int ttad_matrix_get(ttad_matrix* M, double* A, long long lda, int dst_device, void* stream);
+/* Tile-wise copy between matrices of the same partition, converting storage. */
+int ttad_matrix_copy(ttad_matrix* dst, const ttad_matrix* src, void* stream);
+int ttad_matrix_info(const ttad_matrix* M, int* n, int* num_blocks, int* storage, long long* bytes);
+int ttad_matrix_tile(const ttad_matrix* M, int i, int j, void** ptr, int* ld);
+
+/* ---- the tiled Cholesky --------------------------------------------------- */
+/* Replaces cholesky() of 2d_cholesky.cpp:41-690 on one rank, non-accumulate mode:
+ * the whole factorization of M in place (lower tiles become L).
+ * prec: TTAD_F64 (M FP64) or TTAD_F32 / TTAD_BF16 for the trailing update (M FP32;
+ * diagonal POTRF in FP64). prio_kind 0..3 as PrioKind (:39, :157-204).
+ * Synchronous; *info as ttad_dpotrf. */
+int ttad_potrf_tiled(ttad_context* ctx, ttad_matrix* M, int schedule, int prio_kind, int prec, void* stream,
+                     int* info);
+
+/* Device time (HIP events on the factorization's stream) of the last
+ * ttad_potrf_tiled on this context; replaces the t0/t1 wall-clock window of
+ * 2d_cholesky.cpp:611-619. */
+int ttad_last_elapsed(ttad_context* ctx, double* seconds);
+
+/* ---- device task graphs: the Taskflow<K>/Threadpool drop-in ------------------- */
+typedef struct {
+    int32_t kind;  /* TTAD_POTRF .. TTAD_GEMM */
+    int32_t prec;  /* TTAD_F64 / TTAD_F32 / TTAD_BF16 */
+    int32_t mat;   /* matrix slot bound with ttad_graph_bind */
+    int32_t i, j, k;
+} ttad_task_desc;
+
+typedef struct {
+    double elapsed_s;       /* launch -> completion, device events */
+    long long items_done;
+    long long items_total;
+    int workers;            /* persistent workgroups */
+    int abort_code;         /* 0 ok, 1 watchdog, 2 queue overflow */
+} ttad_graph_stats;
+
+/* A graph is the compiled form of the Taskflow callbacks (taskflow.hpp:40-297):
+ * for every task its kind/coordinates (set_task), indegree (set_indegree),
+ * out-edges in CSR form (the fulfill_promise calls of set_fulfill) and priority
+ * (set_priority, higher first). Validated: every edge target must receive exactly
+ * indegree fulfills, counting host seeds. */
+int ttad_graph_create(ttad_context* ctx, int n_tasks, const ttad_task_desc* tasks, const int32_t* indegree,
+                      const int32_t* out_ptr, const int32_t* out_idx, const double* priority, ttad_graph** out);
+int ttad_graph_bind(ttad_graph* g, int slot, ttad_matrix* M);
+/* Taskflow::fulfill_promise(k) called from the host (2d_cholesky.cpp:614). */
+int ttad_graph_seed(ttad_graph* g, int task);
+int ttad_graph_clear_seeds(ttad_graph* g);
+/* Starts the persistent executor (threadpool_shared.cpp:81-198). n_workers <= 0:
+ * all resident slots. trace != 0 records per-item device timestamps. */
+int ttad_graph_launch(ttad_graph* g, int n_workers, int trace, void* stream);
+/* Threadpool::join (threadpool_shared.cpp:234-241): waits for completion,
+ * timeout_s <= 0 waits forever. Returns NOT_SPD if a bound matrix hit a bad pivot. */
+int ttad_graph_wait(ttad_graph* g, double timeout_s, ttad_graph_stats* stats);
+/* Writes the last run's trace in the reference Logger CSV format
+ * (util.cpp:117-127): "<prefix><worker>>run>NAME,start,end" in seconds. */
+int ttad_graph_write_trace(ttad_graph* g, const char* path, const char* prefix);
+int ttad_graph_destroy(ttad_graph* g);
+
+/* The Cholesky PTG of 2d_cholesky.cpp:281-614 (one rank, non-accumulate mode),
+ * declared through the Taskflow drop-in and compiled on the host without a
+ * device: tasks (POTRF/TRSM/SYRK/GEMM with tile coordinates, k = step), their
+ * indegrees, priorities (PrioKind prio_kind) and out-edges in CSR form.
+ * Call once with NULL arrays to get *n_tasks / *n_edges, then with arrays of
+ * at least that size. */
+int ttad_cholesky_dag(int num_blocks, int prio_kind, int* n_tasks, int* n_edges, ttad_task_desc* tasks,
+                      int32_t* indegree, double* priority, int32_t* out_ptr, int32_t* out_idx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TASKTORRENT_AMD_H */

@@ -1,0 +1,255 @@
+"""tasktorrent_amd: MI355X-native tiled dense Cholesky (Python front of the C-ABI).
+
+The product is the HIP library libtasktorrent_amd.so (include/tasktorrent_amd.h);
+this module binds it with ctypes for tests, benchmarks and Python callers. There
+is no CPU fallback: every call goes to the HIP library and fails loudly if the
+library is not built or no GPU is present.
+
+PyTorch is used only as plumbing for device buffers and streams.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+from . import _capi
+from ._capi import (TTAD_BF16, TTAD_F32, TTAD_F64, TTAD_GEMM, TTAD_POTRF, TTAD_SCHED_PERSISTENT,
+                    TTAD_SCHED_STREAM, TTAD_SYRK, TTAD_TRSM, TtadError, lib, lib_path)
+
+__all__ = [
+    "Context", "TiledMatrix", "Graph", "TtadError", "lib", "lib_path",
+    "TTAD_F64", "TTAD_F32", "TTAD_BF16", "TTAD_POTRF", "TTAD_TRSM", "TTAD_SYRK", "TTAD_GEMM",
+    "TTAD_SCHED_STREAM", "TTAD_SCHED_PERSISTENT",
+]
+
+PREC = {"f64": TTAD_F64, "f32": TTAD_F32, "bf16": TTAD_BF16}
+
+
+def _stream_of(torch_stream=None):
+    import torch
+    s = torch_stream if torch_stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Context:
+    """A device context (ttad_init). Replaces the reference Threadpool construction."""
+
+    def __init__(self, device: int = 0):
+        self._h = ctypes.c_void_p()
+        _capi.check(lib().ttad_init(device, ctypes.byref(self._h)))
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().ttad_finalize(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        ncu = ctypes.c_int()
+        hbm = ctypes.c_longlong()
+        name = ctypes.create_string_buffer(64)
+        _capi.check(lib().ttad_device_info(self._h, ctypes.byref(ncu), ctypes.byref(hbm), name, 64))
+        return {"n_cu": ncu.value, "hbm_bytes": hbm.value, "arch": name.value.decode()}
+
+    def synchronize(self, stream=None):
+        _capi.check(lib().ttad_synchronize(self._h, stream))
+
+    # ---- tile kernels on torch device tensors (column-major views, see _colmajor) ----
+    def potrf(self, A, n: int, lda: int, stream=None) -> int:
+        info = ctypes.c_int()
+        fn = lib().ttad_dpotrf if A.dtype.itemsize == 8 else lib().ttad_spotrf
+        rc = fn(self._h, n, _ptr(A), lda, ctypes.byref(info), stream or _stream_of())
+        if rc not in (0, _capi.TTAD_ERR_NOT_SPD):
+            _capi.check(rc)
+        return info.value
+
+    def trsm(self, m, n, L, ldl, B, ldb, stream=None):
+        fn = lib().ttad_dtrsm if L.dtype.itemsize == 8 else lib().ttad_strsm
+        _capi.check(fn(self._h, m, n, _ptr(L), ldl, _ptr(B), ldb, stream or _stream_of()))
+
+    def syrk(self, n, k, alpha, A, lda, beta, C, ldc, stream=None):
+        fn = lib().ttad_dsyrk if A.dtype.itemsize == 8 else lib().ttad_ssyrk
+        _capi.check(fn(self._h, n, k, alpha, _ptr(A), lda, beta, _ptr(C), ldc, stream or _stream_of()))
+
+    def gemm(self, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream=None):
+        fn = lib().ttad_dgemm if A.dtype.itemsize == 8 else lib().ttad_sgemm
+        _capi.check(fn(self._h, m, n, k, alpha, _ptr(A), lda, _ptr(B), ldb, beta, _ptr(C), ldc,
+                       stream or _stream_of()))
+
+    def gemm_bf16(self, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, syrk=False, stream=None):
+        _capi.check(lib().ttad_gemm_bf16(self._h, m, n, k, alpha, _ptr(A), lda, _ptr(B), ldb, beta, _ptr(C),
+                                         ldc, int(syrk), stream or _stream_of()))
+
+    # ---- tiled Cholesky ----
+    def potrf_tiled(self, M: "TiledMatrix", schedule=TTAD_SCHED_PERSISTENT, prio_kind: int = 2,
+                    prec: str | int = "f64", stream=None) -> int:
+        p = PREC[prec] if isinstance(prec, str) else prec
+        info = ctypes.c_int()
+        rc = lib().ttad_potrf_tiled(self._h, M.handle, schedule, prio_kind, p, stream, ctypes.byref(info))
+        if rc not in (0, _capi.TTAD_ERR_NOT_SPD):
+            _capi.check(rc)
+        return info.value
+
+
+    def last_elapsed(self) -> float:
+        """Device seconds of the last potrf_tiled (HIP events on its stream)."""
+        t = ctypes.c_double()
+        _capi.check(lib().ttad_last_elapsed(self._h, ctypes.byref(t)))
+        return t.value
+
+
+class TiledMatrix:
+    """Lower tiles of a symmetric matrix in HBM (ttad_matrix)."""
+
+    def __init__(self, ctx: Context, block_sizes, storage: str | int = "f64"):
+        self.ctx = ctx
+        bs = np.ascontiguousarray(block_sizes, dtype=np.int32)
+        self.block_sizes = bs
+        self.n = int(bs.sum())
+        st = PREC[storage] if isinstance(storage, str) else storage
+        self.storage = st
+        self._h = ctypes.c_void_p()
+        _capi.check(lib().ttad_matrix_create(ctx.handle, len(bs), bs.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                             st, ctypes.byref(self._h)))
+
+    @classmethod
+    def uniform(cls, ctx, n: int, b: int, storage="f64"):
+        nb = (n + b - 1) // b
+        sizes = [b] * (nb - 1) + [n - b * (nb - 1)]
+        return cls(ctx, sizes, storage)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().ttad_matrix_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set(self, A, stream=None):
+        """A: numpy (host, column-major n x n FP64) or a torch CUDA FP64 tensor holding A^T row-major."""
+        if isinstance(A, np.ndarray):
+            Af = np.asfortranarray(A, dtype=np.float64)
+            _capi.check(lib().ttad_matrix_set(self._h, Af.ctypes.data_as(ctypes.c_void_p), self.n, 0, stream))
+        else:
+            assert A.is_cuda and A.dtype.itemsize == 8 and A.is_contiguous()
+            _capi.check(lib().ttad_matrix_set(self._h, _ptr(A), A.shape[1], 1, stream or _stream_of()))
+
+    def get(self, out=None, stream=None):
+        """Lower triangle as a host column-major FP64 array (strict upper = 0), or into a torch CUDA tensor."""
+        if out is None:
+            L = np.zeros((self.n, self.n), dtype=np.float64, order="F")
+            _capi.check(lib().ttad_matrix_get(self._h, L.ctypes.data_as(ctypes.c_void_p), self.n, 0, stream))
+            return L
+        _capi.check(lib().ttad_matrix_get(self._h, _ptr(out), out.shape[1], 1, stream or _stream_of()))
+        return out
+
+    def copy_from(self, src: "TiledMatrix", stream=None):
+        _capi.check(lib().ttad_matrix_copy(self._h, src.handle, stream))
+
+    def bytes(self) -> int:
+        b = ctypes.c_longlong()
+        _capi.check(lib().ttad_matrix_info(self._h, None, None, None, ctypes.byref(b)))
+        return b.value
+
+
+class Graph:
+    """A compiled device task graph (ttad_graph): the Taskflow/Threadpool drop-in at the C level."""
+
+    def __init__(self, ctx: Context, tasks, indegree, out_ptr, out_idx, priority=None):
+        n = len(tasks)
+        arr = (_capi.TaskDesc * n)(*[_capi.TaskDesc(*t) for t in tasks])
+        self._keep = [np.ascontiguousarray(indegree, dtype=np.int32), np.ascontiguousarray(out_ptr, dtype=np.int32),
+                      np.ascontiguousarray(out_idx, dtype=np.int32),
+                      np.ascontiguousarray(priority if priority is not None else np.zeros(n), dtype=np.float64)]
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        self._h = ctypes.c_void_p()
+        _capi.check(lib().ttad_graph_create(ctx.handle, n, arr, ip(self._keep[0]), ip(self._keep[1]),
+                                            ip(self._keep[2]) if len(self._keep[2]) else None,
+                                            self._keep[3].ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                            ctypes.byref(self._h)))
+        self.ctx = ctx
+
+    def bind(self, slot: int, M: TiledMatrix):
+        _capi.check(lib().ttad_graph_bind(self._h, slot, M.handle))
+
+    def seed(self, task: int):
+        _capi.check(lib().ttad_graph_seed(self._h, task))
+
+    def launch(self, n_workers: int = 0, trace: bool = False, stream=None):
+        _capi.check(lib().ttad_graph_launch(self._h, n_workers, int(trace), stream))
+
+    def wait(self, timeout_s: float = 120.0):
+        st = _capi.GraphStats()
+        rc = lib().ttad_graph_wait(self._h, timeout_s, ctypes.byref(st))
+        return rc, {"elapsed_s": st.elapsed_s, "items_done": st.items_done, "items_total": st.items_total,
+                    "workers": st.workers, "abort_code": st.abort_code}
+
+    def write_trace(self, path: str, prefix: str = "Wk_"):
+        _capi.check(lib().ttad_graph_write_trace(self._h, path.encode(), prefix.encode()))
+
+    def close(self):
+        if self._h:
+            lib().ttad_graph_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def cholesky_dag(num_blocks: int, prio_kind: int = 2):
+    """The Cholesky PTG compiled on the host (no device): tasks, indegree, priority, CSR out-edges.
+
+    tasks is a list of (kind, prec, mat, i, j, k); kind uses TTAD_POTRF..TTAD_GEMM (SYRK for i == j).
+    """
+    nt, ne = ctypes.c_int(), ctypes.c_int()
+    _capi.check(lib().ttad_cholesky_dag(num_blocks, prio_kind, ctypes.byref(nt), ctypes.byref(ne),
+                                        None, None, None, None, None))
+    n, e = nt.value, ne.value
+    tasks = (_capi.TaskDesc * n)()
+    indeg = np.zeros(n, dtype=np.int32)
+    prio = np.zeros(n, dtype=np.float64)
+    out_ptr = np.zeros(n + 1, dtype=np.int32)
+    out_idx = np.zeros(max(1, e), dtype=np.int32)
+    ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    _capi.check(lib().ttad_cholesky_dag(num_blocks, prio_kind, ctypes.byref(nt), ctypes.byref(ne), tasks, ip(indeg),
+                                        prio.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ip(out_ptr),
+                                        ip(out_idx)))
+    tl = [(t.kind, t.prec, t.mat, t.i, t.j, t.k) for t in tasks]
+    return tl, indeg, prio, out_ptr, out_idx[:e]
+
+
+def task_name(t) -> str:
+    """Reference task names (2d_cholesky.cpp:206-217): POTRF_j, TRSM_i_j, GEMM_k_i_j."""
+    kind, _, _, i, j, k = t
+    if kind == TTAD_POTRF:
+        return f"POTRF_{j}"
+    if kind == TTAD_TRSM:
+        return f"TRSM_{i}_{j}"
+    return f"GEMM_{k}_{i}_{j}"

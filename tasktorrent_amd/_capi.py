@@ -1,0 +1,136 @@
+"""ctypes declarations of include/tasktorrent_amd.h."""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+_PKG = Path(__file__).resolve().parent
+_LIB = _PKG / "_lib" / "libtasktorrent_amd.so"
+HEADER = _PKG.parent / "include" / "tasktorrent_amd.h"
+
+TTAD_OK = 0
+TTAD_ERR_INVALID = 1
+TTAD_ERR_OOM = 2
+TTAD_ERR_NOT_SPD = 3
+TTAD_ERR_TIMEOUT = 4
+TTAD_ERR_DEVICE = 5
+TTAD_ERR_IR_STALLED = 6
+TTAD_ERR_GRAPH = 7
+TTAD_ERR_BUSY = 8
+TTAD_ERR_NOT_FOUND = 9
+
+TTAD_F64, TTAD_F32, TTAD_BF16 = 0, 1, 2
+TTAD_POTRF, TTAD_TRSM, TTAD_SYRK, TTAD_GEMM = 0, 1, 2, 3
+TTAD_SCHED_STREAM, TTAD_SCHED_PERSISTENT = 0, 1
+
+_NAMES = {0: "OK", 1: "INVALID", 2: "OOM", 3: "NOT_SPD", 4: "TIMEOUT", 5: "DEVICE", 6: "IR_STALLED", 7: "GRAPH",
+          8: "BUSY", 9: "NOT_FOUND"}
+
+
+class TtadError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"TTAD_ERR_{_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class TaskDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("prec", ctypes.c_int32), ("mat", ctypes.c_int32),
+                ("i", ctypes.c_int32), ("j", ctypes.c_int32), ("k", ctypes.c_int32)]
+
+
+class GraphStats(ctypes.Structure):
+    _fields_ = [("elapsed_s", ctypes.c_double), ("items_done", ctypes.c_longlong),
+                ("items_total", ctypes.c_longlong), ("workers", ctypes.c_int), ("abort_code", ctypes.c_int)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("fell_back", ctypes.c_int), ("backward_error", ctypes.c_double),
+                ("factor_s", ctypes.c_double), ("solve_s", ctypes.c_double), ("total_s", ctypes.c_double),
+                ("info", ctypes.c_int), ("prec_used", ctypes.c_int)]
+
+
+_lib = None
+
+V = ctypes.c_void_p
+I = ctypes.c_int
+D = ctypes.c_double
+F = ctypes.c_float
+LL = ctypes.c_longlong
+PI = ctypes.POINTER(ctypes.c_int)
+
+_SIGS = {
+    "ttad_last_error": (ctypes.c_char_p, []),
+    "ttad_version": (I, []),
+    "ttad_init": (I, [I, ctypes.POINTER(V)]),
+    "ttad_finalize": (I, [V]),
+    "ttad_device_info": (I, [V, PI, ctypes.POINTER(LL), ctypes.c_char_p, I]),
+    "ttad_synchronize": (I, [V, V]),
+    "ttad_dpotrf": (I, [V, I, V, I, PI, V]),
+    "ttad_spotrf": (I, [V, I, V, I, PI, V]),
+    "ttad_dtrsm": (I, [V, I, I, V, I, V, I, V]),
+    "ttad_strsm": (I, [V, I, I, V, I, V, I, V]),
+    "ttad_dsyrk": (I, [V, I, I, D, V, I, D, V, I, V]),
+    "ttad_ssyrk": (I, [V, I, I, F, V, I, F, V, I, V]),
+    "ttad_dgemm": (I, [V, I, I, I, D, V, I, V, I, D, V, I, V]),
+    "ttad_sgemm": (I, [V, I, I, I, F, V, I, V, I, F, V, I, V]),
+    "ttad_gemm_bf16": (I, [V, I, I, I, F, V, I, V, I, F, V, I, I, V]),
+    "ttad_matrix_create": (I, [V, I, PI, I, ctypes.POINTER(V)]),
+    "ttad_matrix_destroy": (I, [V]),
+    "ttad_matrix_set": (I, [V, V, LL, I, V]),
+    "ttad_matrix_get": (I, [V, V, LL, I, V]),
+    "ttad_matrix_copy": (I, [V, V, V]),
+    "ttad_matrix_info": (I, [V, PI, PI, PI, ctypes.POINTER(LL)]),
+    "ttad_matrix_tile": (I, [V, I, I, ctypes.POINTER(V), PI]),
+    "ttad_potrf_tiled": (I, [V, V, I, I, I, V, PI]),
+    "ttad_last_elapsed": (I, [V, ctypes.POINTER(D)]),
+    "ttad_graph_create": (I, [V, I, ctypes.POINTER(TaskDesc), ctypes.POINTER(ctypes.c_int32),
+                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(V)]),
+    "ttad_graph_bind": (I, [V, I, V]),
+    "ttad_graph_seed": (I, [V, I]),
+    "ttad_graph_clear_seeds": (I, [V]),
+    "ttad_graph_launch": (I, [V, I, I, V]),
+    "ttad_graph_wait": (I, [V, D, ctypes.POINTER(GraphStats)]),
+    "ttad_graph_write_trace": (I, [V, ctypes.c_char_p, ctypes.c_char_p]),
+    "ttad_graph_destroy": (I, [V]),
+    "ttad_cholesky_dag": (I, [I, I, PI, PI, ctypes.POINTER(TaskDesc), ctypes.POINTER(ctypes.c_int32),
+                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32),
+                              ctypes.POINTER(ctypes.c_int32)]),
+}
+
+
+def lib_path() -> Path:
+    return Path(os.environ.get("TTAD_LIB", _LIB))
+
+
+def lib():
+    """Load the HIP library; raises if it was not built (there is no fallback path)."""
+    global _lib
+    if _lib is None:
+        p = lib_path()
+        if not p.exists():
+            raise RuntimeError(f"tasktorrent_amd: HIP library not built ({p}); run __graft_entry__.build()")
+        L = ctypes.CDLL(str(p))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int):
+    if rc != TTAD_OK:
+        raise TtadError(rc, lib().ttad_last_error().decode(errors="replace"))
+    return rc
+
+
+def declared_symbols(header: Path | None = None) -> list[str]:
+    """Function names declared in a public header (for the export check)."""
+    import re
+    text = (header or HEADER).read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ttad_[a-z0-9_]+)\s*\(", text)))

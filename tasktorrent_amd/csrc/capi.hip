@@ -1,0 +1,757 @@
+// C-ABI of tasktorrent_amd (declared in include/tasktorrent_amd.h).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <numeric>
+#include <thread>
+
+#include "runtime.hpp"
+
+namespace ttad {
+
+static thread_local std::string g_err;
+
+int set_error(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+static hipStream_t pick(ttad_context* c, void* s) { return s ? (hipStream_t)s : c->stream; }
+
+// ---------------------------------------------------------------------------
+// Host-sequenced schedule: per panel step k one launch each for POTRF(k), all
+// TRSM(i,k) and all SYRK/GEMM(k,i,j); stream order carries the dependencies.
+// ---------------------------------------------------------------------------
+struct StreamPlan {
+    struct Batch {
+        int task_off, prefix_off, n_tasks, n_items;
+    };
+    std::vector<Batch> batches;
+    DevBuf d_tasks, d_prefix;
+};
+
+static std::shared_ptr<StreamPlan> build_stream_plan(ttad_matrix* M, int prec, int& rc) {
+    auto plan = std::make_shared<StreamPlan>();
+    std::vector<DevTask> tasks;
+    std::vector<int32_t> prefix;
+    const int nt = M->nt;
+    auto add_batch = [&](const std::vector<DevTask>& ts) {
+        if (ts.empty()) return;
+        StreamPlan::Batch b{(int)tasks.size(), (int)prefix.size(), (int)ts.size(), 0};
+        for (auto& t : ts) {
+            prefix.push_back(b.n_items);
+            b.n_items += t.n_items;
+            tasks.push_back(t);
+        }
+        plan->batches.push_back(b);
+    };
+    auto mk = [&](int kind, int i, int j, int k) {
+        DevTask t{};
+        t.kind = kind;
+        t.prec = (kind == K_GEMM || kind == K_SYRK) ? prec : M->storage;
+        t.mat = 0;
+        t.i = i;
+        t.j = j;
+        t.k = k;
+        t.n_items = std::max(1, items_of(kind, M->bsz[i], M->bsz[j]));
+        t.bucket = 0;
+        return t;
+    };
+    for (int k = 0; k < nt; k++) {
+        add_batch({mk(K_POTRF, k, k, k)});
+        std::vector<DevTask> tr;
+        for (int i = k + 1; i < nt; i++) tr.push_back(mk(K_TRSM, i, k, k));
+        add_batch(tr);
+        std::vector<DevTask> up;
+        for (int j = k + 1; j < nt; j++)
+            for (int i = j; i < nt; i++) up.push_back(mk(i == j ? K_SYRK : K_GEMM, i, j, k));
+        add_batch(up);
+    }
+    rc = TTAD_OK;
+    if (plan->d_tasks.alloc(std::max<size_t>(1, tasks.size()) * sizeof(DevTask)) != hipSuccess ||
+        plan->d_prefix.alloc(std::max<size_t>(1, prefix.size()) * sizeof(int32_t)) != hipSuccess) {
+        rc = set_error(TTAD_ERR_OOM, "stream plan allocation failed");
+        return nullptr;
+    }
+    if (hipMemcpy(plan->d_tasks.p, tasks.data(), tasks.size() * sizeof(DevTask), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(plan->d_prefix.p, prefix.data(), prefix.size() * sizeof(int32_t), hipMemcpyHostToDevice) !=
+            hipSuccess) {
+        rc = set_error(TTAD_ERR_DEVICE, "stream plan upload failed");
+        return nullptr;
+    }
+    return plan;
+}
+
+int cholesky_persistent(ttad_context* ctx, ttad_matrix* M, int prio_kind, int prec, hipStream_t s, int* info);
+
+}  // namespace ttad
+
+using namespace ttad;
+
+extern "C" {
+
+const char* ttad_last_error(void) { return g_err.c_str(); }
+int ttad_version(void) { return 1; }
+
+int ttad_init(int device, ttad_context** out) {
+    if (!out) return set_error(TTAD_ERR_INVALID, "out is NULL");
+    int ndev = 0;
+    TTAD_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return set_error(TTAD_ERR_INVALID, "device %d not present (%d)", device, ndev);
+    TTAD_HIP(hipSetDevice(device));
+    hipDeviceProp_t p;
+    TTAD_HIP(hipGetDeviceProperties(&p, device));
+    auto* c = new ttad_context();
+    c->device = device;
+    c->n_cu = p.multiProcessorCount;
+    c->hbm_bytes = (long long)p.totalGlobalMem;
+    c->name = p.gcnArchName;
+    hipError_t e = hipStreamCreate(&c->stream);
+    if (e == hipSuccess) e = kernels_init();
+    if (e == hipSuccess) e = c->scratch_info.alloc(sizeof(int));
+    if (e != hipSuccess) {
+        delete c;
+        return set_error(TTAD_ERR_DEVICE, "init: %s", hipGetErrorString(e));
+    }
+    c->workers_per_cu = persistent_blocks_per_cu();
+    if (hipEventCreate(&c->ev_a) != hipSuccess || hipEventCreate(&c->ev_b) != hipSuccess) {
+        delete c;
+        return set_error(TTAD_ERR_DEVICE, "init: event creation failed");
+    }
+    *out = c;
+    return TTAD_OK;
+}
+
+int ttad_finalize(ttad_context* ctx) {
+    if (!ctx) return TTAD_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->chol_graphs) ttad_graph_destroy(kv.second);
+    ctx->chol_graphs.clear();
+    ctx->stream_plans.clear();
+    (void)hipStreamDestroy(ctx->stream);
+    if (ctx->ev_a) (void)hipEventDestroy(ctx->ev_a);
+    if (ctx->ev_b) (void)hipEventDestroy(ctx->ev_b);
+    delete ctx;
+    return TTAD_OK;
+}
+
+int ttad_device_info(ttad_context* ctx, int* n_cu, long long* hbm, char* name, int name_len) {
+    if (!ctx) return set_error(TTAD_ERR_INVALID, "ctx is NULL");
+    if (n_cu) *n_cu = ctx->n_cu;
+    if (hbm) *hbm = ctx->hbm_bytes;
+    if (name && name_len > 0) {
+        strncpy(name, ctx->name.c_str(), name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    return TTAD_OK;
+}
+
+int ttad_synchronize(ttad_context* ctx, void* stream) {
+    if (!ctx) return set_error(TTAD_ERR_INVALID, "ctx is NULL");
+    TTAD_HIP(hipStreamSynchronize(pick(ctx, stream)));
+    return TTAD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Tile kernels
+// ---------------------------------------------------------------------------
+static int ensure_dinv(ttad_context* ctx, int n) {
+    const size_t need = (size_t)((n + DB - 1) / DB) * DB * DB * sizeof(double);
+    if (ctx->scratch_dinv.bytes < need) TTAD_HIP(ctx->scratch_dinv.alloc(need));
+    return TTAD_OK;
+}
+
+static int potrf_common(ttad_context* ctx, int prec, int n, void* A, int lda, int* info, void* stream) {
+    if (!ctx || (!A && n > 0) || n < 0 || lda < std::max(1, n)) return set_error(TTAD_ERR_INVALID, "potrf: bad args");
+    if (info) *info = 0;
+    if (n == 0) return TTAD_OK;
+    int rc = ensure_dinv(ctx, n);
+    if (rc) return rc;
+    hipStream_t s = pick(ctx, stream);
+    TTAD_HIP(hipMemsetAsync(ctx->scratch_info.p, 0, sizeof(int), s));
+    launch_raw_potrf(prec, A, lda, n, ctx->scratch_dinv.as<double>(), ctx->scratch_info.as<int>(), s);
+    TTAD_HIP(hipGetLastError());
+    int h = 0;
+    TTAD_HIP(hipMemcpyAsync(&h, ctx->scratch_info.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    TTAD_HIP(hipStreamSynchronize(s));
+    if (info) *info = h;
+    return h ? set_error(TTAD_ERR_NOT_SPD, "potrf: non-positive pivot at column %d", h) : TTAD_OK;
+}
+
+static int trsm_common(ttad_context* ctx, int prec, int m, int n, const void* L, int ldl, void* B, int ldb,
+                       void* stream) {
+    if (!ctx || m < 0 || n < 0 || ldl < std::max(1, n) || ldb < std::max(1, m))
+        return set_error(TTAD_ERR_INVALID, "trsm: bad args");
+    if (m == 0 || n == 0) return TTAD_OK;
+    int rc = ensure_dinv(ctx, n);
+    if (rc) return rc;
+    hipStream_t s = pick(ctx, stream);
+    launch_trtri_diag(prec, L, ldl, n, ctx->scratch_dinv.as<double>(), s);
+    launch_raw_trsm(prec, L, ldl, ctx->scratch_dinv.as<double>(), B, ldb, m, n, s);
+    TTAD_HIP(hipGetLastError());
+    return TTAD_OK;
+}
+
+static int gemm_common(ttad_context* ctx, int prec, int m, int n, int k, double alpha, const void* A, int lda,
+                       const void* B, int ldb, double beta, void* C, int ldc, int syrk, void* stream) {
+    if (!ctx || m < 0 || n < 0 || k < 0 || lda < std::max(1, m) || ldb < std::max(1, n) || ldc < std::max(1, m))
+        return set_error(TTAD_ERR_INVALID, "gemm/syrk: bad args");
+    if (m == 0 || n == 0) return TTAD_OK;
+    launch_raw_gemm(prec, A, lda, B, ldb, C, ldc, m, n, k, syrk, alpha, beta, pick(ctx, stream));
+    TTAD_HIP(hipGetLastError());
+    return TTAD_OK;
+}
+
+int ttad_dpotrf(ttad_context* ctx, int n, double* A, int lda, int* info, void* stream) {
+    return potrf_common(ctx, P_F64, n, A, lda, info, stream);
+}
+int ttad_spotrf(ttad_context* ctx, int n, float* A, int lda, int* info, void* stream) {
+    return potrf_common(ctx, P_F32, n, A, lda, info, stream);
+}
+int ttad_dtrsm(ttad_context* ctx, int m, int n, const double* L, int ldl, double* B, int ldb, void* stream) {
+    return trsm_common(ctx, P_F64, m, n, L, ldl, B, ldb, stream);
+}
+int ttad_strsm(ttad_context* ctx, int m, int n, const float* L, int ldl, float* B, int ldb, void* stream) {
+    return trsm_common(ctx, P_F32, m, n, L, ldl, B, ldb, stream);
+}
+int ttad_dsyrk(ttad_context* ctx, int n, int k, double alpha, const double* A, int lda, double beta, double* C,
+               int ldc, void* stream) {
+    return gemm_common(ctx, P_F64, n, n, k, alpha, A, lda, A, lda, beta, C, ldc, 1, stream);
+}
+int ttad_ssyrk(ttad_context* ctx, int n, int k, float alpha, const float* A, int lda, float beta, float* C, int ldc,
+               void* stream) {
+    return gemm_common(ctx, P_F32, n, n, k, alpha, A, lda, A, lda, beta, C, ldc, 1, stream);
+}
+int ttad_dgemm(ttad_context* ctx, int m, int n, int k, double alpha, const double* A, int lda, const double* B,
+               int ldb, double beta, double* C, int ldc, void* stream) {
+    return gemm_common(ctx, P_F64, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 0, stream);
+}
+int ttad_sgemm(ttad_context* ctx, int m, int n, int k, float alpha, const float* A, int lda, const float* B, int ldb,
+               float beta, float* C, int ldc, void* stream) {
+    return gemm_common(ctx, P_F32, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 0, stream);
+}
+int ttad_gemm_bf16(ttad_context* ctx, int m, int n, int k, float alpha, const float* A, int lda, const float* B,
+                   int ldb, float beta, float* C, int ldc, int syrk, void* stream) {
+    if (syrk && m != n) return set_error(TTAD_ERR_INVALID, "gemm_bf16: syrk needs m == n");
+    return gemm_common(ctx, P_BF16, m, n, k, alpha, A, lda, syrk ? A : B, syrk ? lda : ldb, beta, C, ldc, syrk,
+                       stream);
+}
+
+// ---------------------------------------------------------------------------
+// Tiled matrices
+// ---------------------------------------------------------------------------
+int ttad_matrix_create(ttad_context* ctx, int num_blocks, const int* block_sizes, int storage, ttad_matrix** out) {
+    if (!ctx || !out || num_blocks <= 0 || !block_sizes || (storage != TTAD_F64 && storage != TTAD_F32))
+        return set_error(TTAD_ERR_INVALID, "matrix_create: bad args");
+    auto M = std::make_unique<ttad_matrix>();
+    M->ctx = ctx;
+    M->id = ctx->next_matrix_id++;
+    M->nt = num_blocks;
+    M->storage = storage;
+    M->elem = storage == TTAD_F64 ? 8 : 4;
+    M->bsz.assign(block_sizes, block_sizes + num_blocks);
+    M->boff.assign(num_blocks + 1, 0);
+    for (int i = 0; i < num_blocks; i++) {
+        if (block_sizes[i] < 0 || block_sizes[i] > 4096)
+            return set_error(TTAD_ERR_INVALID, "matrix_create: block size %d out of [0, 4096]", block_sizes[i]);
+        M->boff[i + 1] = M->boff[i] + block_sizes[i];
+    }
+    M->n = M->boff[num_blocks];
+    const int nt = num_blocks;
+    const size_t align = 256 / M->elem;
+    M->tile_off.assign((size_t)nt * nt, 0);
+    size_t off = 0;
+    for (int j = 0; j < nt; j++)
+        for (int i = j; i < nt; i++) {
+            M->tile_off[i + (size_t)j * nt] = off;
+            off += ((size_t)M->bsz[i] * M->bsz[j] + align - 1) / align * align;
+        }
+    size_t dinv_per = 0;
+    std::vector<size_t> dinv_off(nt);
+    for (int j = 0; j < nt; j++) {
+        dinv_off[j] = dinv_per;
+        dinv_per += (size_t)((M->bsz[j] + DB - 1) / DB) * DB * DB;
+    }
+    TTAD_HIP(hipSetDevice(ctx->device));
+    if (M->tiles.alloc(off * M->elem) != hipSuccess || M->dinv.alloc(std::max<size_t>(1, dinv_per) * 8) != hipSuccess)
+        return set_error(TTAD_ERR_OOM, "matrix_create: %zu bytes of tiles do not fit", off * M->elem);
+    M->bytes = (long long)(off * M->elem + dinv_per * 8);
+    std::vector<void*> tp((size_t)nt * nt, nullptr);
+    std::vector<double*> dp(nt);
+    for (int j = 0; j < nt; j++) {
+        dp[j] = M->dinv.as<double>() + dinv_off[j];
+        for (int i = j; i < nt; i++)
+            tp[i + (size_t)j * nt] = (char*)M->tiles.p + M->tile_off[i + (size_t)j * nt] * M->elem;
+    }
+    TTAD_HIP(M->tab_bsz.alloc(nt * 4));
+    TTAD_HIP(M->tab_boff.alloc((nt + 1) * 4));
+    TTAD_HIP(M->tab_tiles.alloc((size_t)nt * nt * sizeof(void*)));
+    TTAD_HIP(M->tab_dinv.alloc(nt * sizeof(double*)));
+    TTAD_HIP(M->info.alloc(sizeof(int)));
+    TTAD_HIP(M->desc.alloc(sizeof(DevMat)));
+    TTAD_HIP(hipMemcpy(M->tab_bsz.p, M->bsz.data(), nt * 4, hipMemcpyHostToDevice));
+    TTAD_HIP(hipMemcpy(M->tab_boff.p, M->boff.data(), (nt + 1) * 4, hipMemcpyHostToDevice));
+    TTAD_HIP(hipMemcpy(M->tab_tiles.p, tp.data(), tp.size() * sizeof(void*), hipMemcpyHostToDevice));
+    TTAD_HIP(hipMemcpy(M->tab_dinv.p, dp.data(), dp.size() * sizeof(double*), hipMemcpyHostToDevice));
+    TTAD_HIP(hipMemset(M->info.p, 0, sizeof(int)));
+    TTAD_HIP(hipMemset(M->tiles.p, 0, off * M->elem));
+    DevMat d{};
+    d.nt = nt;
+    d.storage = storage;
+    d.bsz = M->tab_bsz.as<int32_t>();
+    d.boff = M->tab_boff.as<int32_t>();
+    d.tiles = M->tab_tiles.as<void*>();
+    d.dinv = M->tab_dinv.as<double*>();
+    d.info = M->info.as<int32_t>();
+    M->host_desc = d;
+    TTAD_HIP(hipMemcpy(M->desc.p, &d, sizeof(DevMat), hipMemcpyHostToDevice));
+    *out = M.release();
+    return TTAD_OK;
+}
+
+int ttad_matrix_destroy(ttad_matrix* M) {
+    if (!M) return TTAD_OK;
+    ttad_context* ctx = M->ctx;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto it = ctx->stream_plans.begin(); it != ctx->stream_plans.end();)
+        it = (it->first.first == M->id) ? ctx->stream_plans.erase(it) : std::next(it);
+    for (auto it = ctx->chol_graphs.begin(); it != ctx->chol_graphs.end();) {
+        if (std::get<0>(it->first) == M->id) {
+            ttad_graph_destroy(it->second);
+            it = ctx->chol_graphs.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    delete M;
+    return TTAD_OK;
+}
+
+int ttad_matrix_set(ttad_matrix* M, const double* A, long long lda, int src_device, void* stream) {
+    if (!M || !A || lda < std::max(1, M->n)) return set_error(TTAD_ERR_INVALID, "matrix_set: bad args");
+    hipStream_t s = pick(M->ctx, stream);
+    if (src_device) {
+        launch_pack(M->storage, A, lda, M->desc.as<DevMat>(), M->nt, s);
+        TTAD_HIP(hipGetLastError());
+        return TTAD_OK;
+    }
+    DevBuf tmp;
+    if (tmp.alloc((size_t)M->n * M->n * 8) != hipSuccess) return set_error(TTAD_ERR_OOM, "matrix_set: staging");
+    TTAD_HIP(hipMemcpy2DAsync(tmp.p, (size_t)M->n * 8, A, (size_t)lda * 8, (size_t)M->n * 8, M->n,
+                              hipMemcpyHostToDevice, s));
+    launch_pack(M->storage, tmp.as<double>(), M->n, M->desc.as<DevMat>(), M->nt, s);
+    TTAD_HIP(hipGetLastError());
+    TTAD_HIP(hipStreamSynchronize(s));
+    return TTAD_OK;
+}
+
+int ttad_matrix_get(ttad_matrix* M, double* A, long long lda, int dst_device, void* stream) {
+    if (!M || !A || lda < std::max(1, M->n)) return set_error(TTAD_ERR_INVALID, "matrix_get: bad args");
+    hipStream_t s = pick(M->ctx, stream);
+    if (dst_device) {
+        launch_unpack(M->storage, A, lda, M->desc.as<DevMat>(), M->nt, 1, s);
+        TTAD_HIP(hipGetLastError());
+        return TTAD_OK;
+    }
+    DevBuf tmp;
+    if (tmp.alloc((size_t)M->n * M->n * 8) != hipSuccess) return set_error(TTAD_ERR_OOM, "matrix_get: staging");
+    launch_unpack(M->storage, tmp.as<double>(), M->n, M->desc.as<DevMat>(), M->nt, 1, s);
+    TTAD_HIP(hipGetLastError());
+    TTAD_HIP(hipMemcpy2DAsync(A, (size_t)lda * 8, tmp.p, (size_t)M->n * 8, (size_t)M->n * 8, M->n,
+                              hipMemcpyDeviceToHost, s));
+    TTAD_HIP(hipStreamSynchronize(s));
+    return TTAD_OK;
+}
+
+int ttad_matrix_copy(ttad_matrix* dst, const ttad_matrix* src, void* stream) {
+    if (!dst || !src || dst->nt != src->nt || dst->bsz != src->bsz)
+        return set_error(TTAD_ERR_INVALID, "matrix_copy: partitions differ");
+    launch_copy_tiles(dst->storage, src->storage, src->desc.as<DevMat>(), dst->desc.as<DevMat>(), dst->nt,
+                      pick(dst->ctx, stream));
+    TTAD_HIP(hipGetLastError());
+    return TTAD_OK;
+}
+
+int ttad_matrix_info(const ttad_matrix* M, int* n, int* num_blocks, int* storage, long long* bytes) {
+    if (!M) return set_error(TTAD_ERR_INVALID, "matrix is NULL");
+    if (n) *n = M->n;
+    if (num_blocks) *num_blocks = M->nt;
+    if (storage) *storage = M->storage;
+    if (bytes) *bytes = M->bytes;
+    return TTAD_OK;
+}
+
+int ttad_matrix_tile(const ttad_matrix* M, int i, int j, void** ptr, int* ld) {
+    if (!M || i < 0 || j < 0 || i >= M->nt || j > i) return set_error(TTAD_ERR_INVALID, "matrix_tile: (%d,%d)", i, j);
+    if (ptr) *ptr = (char*)M->tiles.p + M->tile_off[i + (size_t)j * M->nt] * M->elem;
+    if (ld) *ld = std::max(1, M->bsz[i]);
+    return TTAD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Tiled Cholesky
+// ---------------------------------------------------------------------------
+int ttad_potrf_tiled(ttad_context* ctx, ttad_matrix* M, int schedule, int prio_kind, int prec, void* stream,
+                     int* info) {
+    if (!ctx || !M || prio_kind < 0 || prio_kind > 3) return set_error(TTAD_ERR_INVALID, "potrf_tiled: bad args");
+    if ((prec == TTAD_F64) != (M->storage == TTAD_F64) || prec < 0 || prec > 2)
+        return set_error(TTAD_ERR_INVALID, "potrf_tiled: precision %d does not match the tile storage", prec);
+    if (info) *info = 0;
+    hipStream_t s = pick(ctx, stream);
+    TTAD_HIP(hipMemsetAsync(M->info.p, 0, sizeof(int), s));
+    int rc = TTAD_OK;
+    ctx->ev_valid = false;
+    TTAD_HIP(hipEventRecord(ctx->ev_a, s));
+    if (schedule == TTAD_SCHED_STREAM) {
+        auto key = std::make_pair(M->id, prec);
+        auto it = ctx->stream_plans.find(key);
+        std::shared_ptr<StreamPlan> plan;
+        if (it == ctx->stream_plans.end()) {
+            plan = build_stream_plan(M, prec, rc);
+            if (!plan) return rc;
+            ctx->stream_plans[key] = plan;
+        } else {
+            plan = it->second;
+        }
+        for (auto& b : plan->batches)
+            launch_items(M->desc.as<DevMat>(), plan->d_tasks.as<DevTask>() + b.task_off,
+                         plan->d_prefix.as<int32_t>() + b.prefix_off, b.n_tasks, b.n_items, s);
+        TTAD_HIP(hipGetLastError());
+    } else if (schedule == TTAD_SCHED_PERSISTENT) {
+        rc = cholesky_persistent(ctx, M, prio_kind, prec, s, info);
+        if (rc != TTAD_OK && rc != TTAD_ERR_NOT_SPD) return rc;
+    } else {
+        return set_error(TTAD_ERR_INVALID, "potrf_tiled: unknown schedule %d", schedule);
+    }
+    TTAD_HIP(hipEventRecord(ctx->ev_b, s));
+    ctx->ev_valid = true;
+    int h = 0;
+    TTAD_HIP(hipMemcpyAsync(&h, M->info.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    TTAD_HIP(hipStreamSynchronize(s));
+    if (info) *info = h;
+    if (h) return set_error(TTAD_ERR_NOT_SPD, "potrf_tiled: non-positive pivot at column %d", h);
+    return TTAD_OK;
+}
+
+int ttad_last_elapsed(ttad_context* ctx, double* seconds) {
+    if (!ctx || !seconds) return set_error(TTAD_ERR_INVALID, "last_elapsed: bad args");
+    if (!ctx->ev_valid) return set_error(TTAD_ERR_INVALID, "last_elapsed: nothing timed yet");
+    TTAD_HIP(hipEventSynchronize(ctx->ev_b));
+    float ms = 0.f;
+    TTAD_HIP(hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b));
+    *seconds = ms * 1e-3;
+    return TTAD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Device task graphs
+// ---------------------------------------------------------------------------
+int ttad_graph_create(ttad_context* ctx, int n_tasks, const ttad_task_desc* tasks, const int32_t* indegree,
+                      const int32_t* out_ptr, const int32_t* out_idx, const double* priority, ttad_graph** out) {
+    if (!ctx || !out || n_tasks <= 0 || !tasks || !indegree || !out_ptr)
+        return set_error(TTAD_ERR_INVALID, "graph_create: bad args");
+    if (n_tasks >= (1 << 23)) return set_error(TTAD_ERR_INVALID, "graph_create: too many tasks");
+    auto g = std::make_unique<ttad_graph>();
+    g->ctx = ctx;
+    g->n_tasks = n_tasks;
+    g->tasks.resize(n_tasks);
+    g->indegree.assign(indegree, indegree + n_tasks);
+    g->out_ptr.assign(out_ptr, out_ptr + n_tasks + 1);
+    if (g->out_ptr[0] != 0) return set_error(TTAD_ERR_INVALID, "graph_create: out_ptr[0] != 0");
+    const int n_edges = g->out_ptr[n_tasks];
+    if (n_edges < 0 || (n_edges > 0 && !out_idx)) return set_error(TTAD_ERR_INVALID, "graph_create: bad edges");
+    if (n_edges > 0) g->out_idx.assign(out_idx, out_idx + n_edges);
+    for (int t = 0; t < n_tasks; t++) {
+        if (g->out_ptr[t + 1] < g->out_ptr[t]) return set_error(TTAD_ERR_INVALID, "graph_create: out_ptr not sorted");
+        if (indegree[t] < 1) return set_error(TTAD_ERR_GRAPH, "graph_create: task %d has indegree < 1", t);
+        const ttad_task_desc& d = tasks[t];
+        if (d.kind < TTAD_POTRF || d.kind > TTAD_GEMM || d.prec < 0 || d.prec > 2 || d.mat < 0 || d.mat > 65535)
+            return set_error(TTAD_ERR_INVALID, "graph_create: bad descriptor for task %d", t);
+        DevTask& x = g->tasks[t];
+        x.kind = d.kind;
+        x.prec = d.prec;
+        x.mat = d.mat;
+        x.i = d.i;
+        x.j = d.j;
+        x.k = d.k;
+        x.n_items = 1;
+        x.bucket = 0;
+    }
+    for (int e = 0; e < n_edges; e++)
+        if (g->out_idx[e] < 0 || g->out_idx[e] >= n_tasks)
+            return set_error(TTAD_ERR_INVALID, "graph_create: edge target %d out of range", g->out_idx[e]);
+    // priority -> bucket: distinct priorities, highest first, at most 1024 buckets
+    std::vector<double> pr(n_tasks, 0.0);
+    if (priority) pr.assign(priority, priority + n_tasks);
+    std::vector<double> uniq = pr;
+    std::sort(uniq.begin(), uniq.end(), std::greater<double>());
+    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+    const int nu = (int)uniq.size();
+    const int cap = 1024;
+    g->n_buckets = std::min(nu, cap);
+    for (int t = 0; t < n_tasks; t++) {
+        const int rank = (int)(std::lower_bound(uniq.begin(), uniq.end(), pr[t], std::greater<double>()) - uniq.begin());
+        g->tasks[t].bucket = (nu <= cap) ? rank : (int)((long long)rank * cap / nu);
+    }
+    g->seeds.assign(n_tasks, 0);
+    TTAD_HIP(hipEventCreate(&g->ev_start));
+    TTAD_HIP(hipEventCreate(&g->ev_stop));
+    *out = g.release();
+    return TTAD_OK;
+}
+
+int ttad_graph_bind(ttad_graph* g, int slot, ttad_matrix* M) {
+    if (!g || slot < 0 || slot > 65535 || !M) return set_error(TTAD_ERR_INVALID, "graph_bind: bad args");
+    if (g->running) return set_error(TTAD_ERR_BUSY, "graph_bind: graph is running");
+    if ((int)g->mats.size() <= slot) g->mats.resize(slot + 1, nullptr);
+    if (g->mats[slot] != M) g->mats_dirty = true;
+    g->mats[slot] = M;
+    return TTAD_OK;
+}
+
+int ttad_graph_seed(ttad_graph* g, int task) {
+    if (!g || task < 0 || task >= g->n_tasks) return set_error(TTAD_ERR_INVALID, "graph_seed: bad task");
+    if (g->running) return set_error(TTAD_ERR_BUSY, "graph_seed: graph is running");
+    g->seeds[task]++;
+    g->seeds_dirty = true;
+    return TTAD_OK;
+}
+
+int ttad_graph_clear_seeds(ttad_graph* g) {
+    if (!g) return set_error(TTAD_ERR_INVALID, "graph is NULL");
+    std::fill(g->seeds.begin(), g->seeds.end(), 0);
+    g->seeds_dirty = true;
+    return TTAD_OK;
+}
+
+static int graph_prepare(ttad_graph* g) {
+    const int n = g->n_tasks;
+    if (g->mats_dirty) {
+        for (int t = 0; t < n; t++) {
+            DevTask& x = g->tasks[t];
+            if (x.mat >= (int)g->mats.size() || !g->mats[x.mat])
+                return set_error(TTAD_ERR_INVALID, "graph: task %d uses unbound matrix slot %d", t, x.mat);
+            ttad_matrix* M = g->mats[x.mat];
+            if (x.i < 0 || x.j < 0 || x.k < 0 || x.i >= M->nt || x.j >= M->nt || x.k >= M->nt || x.j > x.i)
+                return set_error(TTAD_ERR_INVALID, "graph: task %d tile (%d,%d,%d) outside the matrix", t, x.i, x.j,
+                                 x.k);
+            if ((x.kind == K_GEMM || x.kind == K_SYRK) && ((x.prec == P_F64) != (M->storage == P_F64)))
+                return set_error(TTAD_ERR_INVALID, "graph: task %d precision does not match tile storage", t);
+            x.n_items = std::max(1, items_of(x.kind, M->bsz[x.i], M->bsz[x.j]));
+        }
+        g->bucket_items.assign(g->n_buckets, 0);
+        g->total_items = 0;
+        for (auto& x : g->tasks) {
+            g->bucket_items[x.bucket] += x.n_items;
+            g->total_items += x.n_items;
+        }
+        g->q_base.assign(g->n_buckets + 1, 0);
+        for (int b = 0; b < g->n_buckets; b++) g->q_base[b + 1] = g->q_base[b] + g->bucket_items[b];
+        std::vector<DevMat> dm(g->mats.size());
+        for (size_t s = 0; s < g->mats.size(); s++)
+            if (g->mats[s]) dm[s] = g->mats[s]->host_desc;
+        const int ne = std::max(1, g->out_ptr[n]);
+        TTAD_HIP(g->d_tasks.alloc(n * sizeof(DevTask)));
+        TTAD_HIP(g->d_out_ptr.alloc((n + 1) * 4));
+        TTAD_HIP(g->d_out_idx.alloc(ne * 4));
+        TTAD_HIP(g->d_deps0.alloc(n * 4));
+        TTAD_HIP(g->d_deps.alloc(n * 4));
+        TTAD_HIP(g->d_items_left.alloc(n * 4));
+        TTAD_HIP(g->d_tail0.alloc(g->n_buckets * 4));
+        TTAD_HIP(g->d_q_head.alloc(g->n_buckets * 4));
+        TTAD_HIP(g->d_q_tail.alloc(g->n_buckets * 4));
+        TTAD_HIP(g->d_q_base.alloc((g->n_buckets + 1) * 4));
+        TTAD_HIP(g->d_q_items.alloc((size_t)g->total_items * 8));
+        TTAD_HIP(g->d_done.alloc(4));
+        TTAD_HIP(g->d_abort.alloc(4));
+        TTAD_HIP(g->d_trace_cursor.alloc(4));
+        TTAD_HIP(g->d_mats.alloc(dm.size() * sizeof(DevMat)));
+        TTAD_HIP(hipMemcpy(g->d_tasks.p, g->tasks.data(), n * sizeof(DevTask), hipMemcpyHostToDevice));
+        TTAD_HIP(hipMemcpy(g->d_out_ptr.p, g->out_ptr.data(), (n + 1) * 4, hipMemcpyHostToDevice));
+        if (!g->out_idx.empty())
+            TTAD_HIP(hipMemcpy(g->d_out_idx.p, g->out_idx.data(), g->out_idx.size() * 4, hipMemcpyHostToDevice));
+        TTAD_HIP(hipMemcpy(g->d_q_base.p, g->q_base.data(), (g->n_buckets + 1) * 4, hipMemcpyHostToDevice));
+        TTAD_HIP(hipMemcpy(g->d_mats.p, dm.data(), dm.size() * sizeof(DevMat), hipMemcpyHostToDevice));
+        g->mats_dirty = false;
+        g->seeds_dirty = true;
+    }
+    if (g->seeds_dirty) {
+        // every task must receive exactly indegree promises: edges + host seeds
+        std::vector<int> incoming(n, 0);
+        for (int v : g->out_idx) incoming[v]++;
+        for (int t = 0; t < n; t++)
+            if (incoming[t] + g->seeds[t] != g->indegree[t])
+                return set_error(TTAD_ERR_GRAPH, "graph: task %d has indegree %d but receives %d edges + %d seeds", t,
+                                 g->indegree[t], incoming[t], g->seeds[t]);
+        std::vector<int32_t> deps0(n), tail0(g->n_buckets, 0);
+        std::vector<int64_t> init;
+        for (int t = 0; t < n; t++) {
+            deps0[t] = g->indegree[t] - g->seeds[t];
+            if (deps0[t] == 0) {
+                const DevTask& x = g->tasks[t];
+                for (int r = 0; r < x.n_items; r++) {
+                    const int64_t slot = g->q_base[x.bucket] + tail0[x.bucket]++;
+                    init.push_back((slot << 40) | ((int64_t)t << 20) | r);
+                }
+            }
+        }
+        if (init.empty()) return set_error(TTAD_ERR_GRAPH, "graph: no task is ready (seed one with ttad_graph_seed)");
+        g->n_init = (int)init.size();
+        TTAD_HIP(g->d_init.alloc(init.size() * 8));
+        TTAD_HIP(hipMemcpy(g->d_init.p, init.data(), init.size() * 8, hipMemcpyHostToDevice));
+        TTAD_HIP(hipMemcpy(g->d_deps0.p, deps0.data(), n * 4, hipMemcpyHostToDevice));
+        TTAD_HIP(hipMemcpy(g->d_tail0.p, tail0.data(), g->n_buckets * 4, hipMemcpyHostToDevice));
+        g->seeds_dirty = false;
+    }
+    return TTAD_OK;
+}
+
+static DevGraph graph_image(ttad_graph* g, bool trace, int64_t spin_limit) {
+    DevGraph d{};
+    d.n_tasks = g->n_tasks;
+    d.n_buckets = g->n_buckets;
+    d.total_items = g->total_items;
+    d.n_init = g->n_init;
+    d.tasks = g->d_tasks.as<DevTask>();
+    d.out_ptr = g->d_out_ptr.as<int32_t>();
+    d.out_idx = g->d_out_idx.as<int32_t>();
+    d.deps0 = g->d_deps0.as<int32_t>();
+    d.tail0 = g->d_tail0.as<int32_t>();
+    d.init_items = g->d_init.as<int64_t>();
+    d.deps = g->d_deps.as<int32_t>();
+    d.items_left = g->d_items_left.as<int32_t>();
+    d.q_head = g->d_q_head.as<int32_t>();
+    d.q_tail = g->d_q_tail.as<int32_t>();
+    d.q_base = g->d_q_base.as<int32_t>();
+    d.q_items = g->d_q_items.as<int64_t>();
+    d.done_items = g->d_done.as<int32_t>();
+    d.abort_flag = g->d_abort.as<int32_t>();
+    d.mats = g->d_mats.as<DevMat>();
+    d.trace = trace ? g->d_trace.as<uint64_t>() : nullptr;
+    d.trace_cursor = g->d_trace_cursor.as<int32_t>();
+    d.spin_limit = spin_limit;
+    return d;
+}
+
+int ttad_graph_launch(ttad_graph* g, int n_workers, int trace, void* stream) {
+    if (!g) return set_error(TTAD_ERR_INVALID, "graph is NULL");
+    if (g->running) return set_error(TTAD_ERR_BUSY, "graph_launch: already running");
+    ttad_context* ctx = g->ctx;
+    int rc = graph_prepare(g);
+    if (rc) return rc;
+    if (trace) TTAD_HIP(g->d_trace.alloc((size_t)g->total_items * 4 * 8));
+    g->traced = trace != 0;
+    int rate_khz = 100000;
+    (void)hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device);
+    const int64_t spin_limit = (int64_t)rate_khz * 1000 * 5;  // 5 s without finding work
+    const DevGraph d = graph_image(g, trace != 0, spin_limit);
+    g->workers = n_workers > 0 ? n_workers : ctx->n_cu * ctx->workers_per_cu;
+    g->stream = pick(ctx, stream);
+    launch_graph_reset(d, g->total_items, g->stream);
+    for (auto* M : g->mats)
+        if (M) TTAD_HIP(hipMemsetAsync(M->info.p, 0, sizeof(int), g->stream));
+    TTAD_HIP(hipEventRecord(g->ev_start, g->stream));
+    launch_persistent(d, g->workers, g->stream);
+    TTAD_HIP(hipGetLastError());
+    TTAD_HIP(hipEventRecord(g->ev_stop, g->stream));
+    g->running = true;
+    return TTAD_OK;
+}
+
+int ttad_graph_wait(ttad_graph* g, double timeout_s, ttad_graph_stats* stats) {
+    if (!g) return set_error(TTAD_ERR_INVALID, "graph is NULL");
+    if (!g->running) return set_error(TTAD_ERR_INVALID, "graph_wait: not launched");
+    auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        hipError_t e = hipEventQuery(g->ev_stop);
+        if (e == hipSuccess) break;
+        if (e != hipErrorNotReady) {
+            g->running = false;
+            return set_error(TTAD_ERR_DEVICE, "graph_wait: %s", hipGetErrorString(e));
+        }
+        if (timeout_s > 0 &&
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > timeout_s)
+            return set_error(TTAD_ERR_TIMEOUT, "graph_wait: timed out after %.1f s (graph still running)", timeout_s);
+        std::this_thread::sleep_for(std::chrono::microseconds(50));
+    }
+    g->running = false;
+    int done = 0, abort_code = 0;
+    TTAD_HIP(hipMemcpy(&done, g->d_done.p, 4, hipMemcpyDeviceToHost));
+    TTAD_HIP(hipMemcpy(&abort_code, g->d_abort.p, 4, hipMemcpyDeviceToHost));
+    float ms = 0.f;
+    TTAD_HIP(hipEventElapsedTime(&ms, g->ev_start, g->ev_stop));
+    if (stats) {
+        stats->elapsed_s = ms * 1e-3;
+        stats->items_done = done;
+        stats->items_total = g->total_items;
+        stats->workers = g->workers;
+        stats->abort_code = abort_code;
+    }
+    if (abort_code)
+        return set_error(abort_code == 1 ? TTAD_ERR_TIMEOUT : TTAD_ERR_DEVICE,
+                         "graph aborted on the device (code %d, %d of %d items done)", abort_code, done,
+                         g->total_items);
+    if (done != g->total_items)
+        return set_error(TTAD_ERR_DEVICE, "graph finished with %d of %d items", done, g->total_items);
+    for (auto* M : g->mats) {
+        if (!M) continue;
+        int h = 0;
+        TTAD_HIP(hipMemcpy(&h, M->info.p, 4, hipMemcpyDeviceToHost));
+        if (h) return set_error(TTAD_ERR_NOT_SPD, "graph: non-positive pivot at column %d", h);
+    }
+    return TTAD_OK;
+}
+
+static std::string task_label(const DevTask& t) {
+    char b[96];
+    if (t.kind == K_POTRF) snprintf(b, sizeof(b), "POTRF_%d", t.j);
+    else if (t.kind == K_TRSM) snprintf(b, sizeof(b), "TRSM_%d_%d", t.i, t.j);
+    else snprintf(b, sizeof(b), "GEMM_%d_%d_%d", t.k, t.i, t.j);
+    return b;
+}
+
+int ttad_graph_write_trace(ttad_graph* g, const char* path, const char* prefix) {
+    if (!g || !path) return set_error(TTAD_ERR_INVALID, "graph_write_trace: bad args");
+    if (!g->traced || g->running) return set_error(TTAD_ERR_INVALID, "graph_write_trace: no finished traced run");
+    std::vector<uint64_t> rec((size_t)g->total_items * 4);
+    int cnt = 0;
+    TTAD_HIP(hipMemcpy(&cnt, g->d_trace_cursor.p, 4, hipMemcpyDeviceToHost));
+    cnt = std::min(cnt, g->total_items);
+    TTAD_HIP(hipMemcpy(rec.data(), g->d_trace.p, (size_t)cnt * 32, hipMemcpyDeviceToHost));
+    int rate_khz = 100000;
+    (void)hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, g->ctx->device);
+    uint64_t t0 = ~0ull;
+    for (int r = 0; r < cnt; r++) t0 = std::min(t0, rec[4 * r + 2]);
+    std::ofstream f(path);
+    if (!f) return set_error(TTAD_ERR_INVALID, "graph_write_trace: cannot open %s", path);
+    f.precision(12);
+    const double tick = 1.0 / ((double)rate_khz * 1000.0);
+    const std::string pre = prefix ? prefix : "Wk_";
+    for (int r = 0; r < cnt; r++) {
+        const uint64_t e = rec[4 * r];
+        const DevTask& t = g->tasks[(size_t)(e >> 20)];
+        f << pre << rec[4 * r + 1] << ">run>" << task_label(t) << "#" << (e & 0xFFFFF) << ","
+          << (double)(rec[4 * r + 2] - t0) * tick << "," << (double)(rec[4 * r + 3] - t0) * tick << "\n";
+    }
+    return TTAD_OK;
+}
+
+int ttad_graph_destroy(ttad_graph* g) {
+    if (!g) return TTAD_OK;
+    if (g->running) (void)hipEventSynchronize(g->ev_stop);
+    if (g->ev_start) (void)hipEventDestroy(g->ev_start);
+    if (g->ev_stop) (void)hipEventDestroy(g->ev_stop);
+    delete g;
+    return TTAD_OK;
+}
+
+}  // extern "C"

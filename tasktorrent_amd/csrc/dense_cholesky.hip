@@ -1,0 +1,137 @@
+// The tiled dense Cholesky PTG of miniapp/dense_cholesky/2d_cholesky.cpp:41-690
+// (one rank, non-accumulate mode), written against the Taskflow<K>/Threadpool
+// drop-in API and compiled into a persistent device graph.
+#include <array>
+#include <memory>
+#include <string>
+
+#include "../../include/tasktorrent_amd/taskflow.hpp"
+#include "runtime.hpp"
+
+namespace ttad {
+
+using int2 = std::array<int, 2>;
+using int3 = std::array<int, 3>;
+
+// Priorities of 2d_cholesky.cpp:157-204 (PrioKind 0 none, 1 row, 2 critical path, 3 cp + row).
+static double prio_potrf(int kind, int nb, int j) {
+    if (kind == 3) return (double)(9 * (nb - j) - 1) + 18.0 * nb * nb;
+    if (kind == 2) return (double)(9 * (nb - j) - 1);
+    if (kind == 1) return 3.0 * (double)(nb - j);
+    return 3.0;
+}
+static double prio_trsm(int kind, int nb, int i, int j) {
+    if (kind == 3) return (double)((nb - i) + nb * (9.0 * nb - 9.0 * j - 2.0) + 9.0 * nb * nb);
+    if (kind == 2) return (double)(9 * (nb - j) - 2);
+    if (kind == 1) return 2.0 * (double)(nb - i);
+    return 2.0;
+}
+static double prio_gemm(int kind, int nb, int k, int i, int j) {
+    if (kind == 3) return (double)(nb - i) + nb * (9.0 * nb - 3.0 * j - 6.0 * k - 2.0);
+    if (kind == 2) return (double)(9 * nb - 9 * j - 2);
+    if (kind == 1) return (double)(nb - i);
+    return 1.0;
+}
+
+// Builds the DAG through the Taskflow API exactly as the reference miniapp declares it.
+// ctx == nullptr: dry compile into `dry` (no device).
+static int build_cholesky_graph(ttad_context* ctx, int nb, int storage, int prec, int prio_kind, ttad_graph** out,
+                                Threadpool** dry = nullptr) {
+    auto tp_owned = std::make_unique<Threadpool>(ctx);
+    Threadpool& tp = *tp_owned;
+    Taskflow<int> potrf(&tp);
+    Taskflow<int2> trsm(&tp);
+    Taskflow<int3> gemm(&tp);
+    auto block_2_thread = [&](int i, int j) { return (i + j * nb) % 1; };
+
+    potrf.set_task([&](int j) { return DeviceTask{TTAD_POTRF, storage, 0, j, j, j}; })
+        .set_fulfill([&](int j) {  // 2d_cholesky.cpp:323-334
+            for (int i = j + 1; i < nb; i++) trsm.fulfill_promise({i, j});
+        })
+        .set_indegree([](int) { return 1; })
+        .set_priority([&](int j) { return prio_potrf(prio_kind, nb, j); })
+        .set_mapping([&](int j) { return block_2_thread(j, j); })
+        .set_name([](int j) { return "POTRF_" + std::to_string(j); });
+
+    trsm.set_task([&](int2 ij) { return DeviceTask{TTAD_TRSM, storage, 0, ij[0], ij[1], ij[1]}; })
+        .set_fulfill([&](int2 ij) {  // 2d_cholesky.cpp:409-431
+            const int i = ij[0], j = ij[1];
+            for (int k = j + 1; k < i; k++) gemm.fulfill_promise({j, i, k});
+            for (int k = i; k < nb; k++) gemm.fulfill_promise({j, k, i});
+        })
+        .set_indegree([](int2 ij) { return 1 + (ij[1] == 0 ? 0 : 1); })
+        .set_priority([&](int2 ij) { return prio_trsm(prio_kind, nb, ij[0], ij[1]); })
+        .set_mapping([&](int2 ij) { return block_2_thread(ij[0], ij[1]); })
+        .set_name([](int2 ij) { return "TRSM_" + std::to_string(ij[0]) + "_" + std::to_string(ij[1]); });
+
+    gemm.set_task([&](int3 kij) {
+            return DeviceTask{kij[1] == kij[2] ? TTAD_SYRK : TTAD_GEMM, prec, 0, kij[1], kij[2], kij[0]};
+        })
+        .set_fulfill([&](int3 kij) {  // 2d_cholesky.cpp:502-531, non-accumulate branch
+            const int k = kij[0], i = kij[1], j = kij[2];
+            if (k < j - 1) gemm.fulfill_promise({k + 1, i, j});
+            else if (i == j) potrf.fulfill_promise(i);
+            else trsm.fulfill_promise({i, j});
+        })
+        .set_indegree([](int3 kij) { return (kij[1] == kij[2] ? 1 : 2) + (kij[0] == 0 ? 0 : 1); })
+        .set_priority([&](int3 kij) { return prio_gemm(prio_kind, nb, kij[0], kij[1], kij[2]); })
+        .set_mapping([&](int3 kij) { return block_2_thread(kij[1], kij[2]); })
+        .set_binding([](int3) { return false; })
+        .set_name([](int3 kij) {
+            return "GEMM_" + std::to_string(kij[0]) + "_" + std::to_string(kij[1]) + "_" + std::to_string(kij[2]);
+        });
+
+    potrf.fulfill_promise(0);  // 2d_cholesky.cpp:614
+    const int rc = tp.compile();
+    if (rc != TTAD_OK) return set_error(rc, "cholesky graph: %s", tp.error().c_str());
+    if (out) *out = tp.release_graph();
+    if (dry) *dry = tp_owned.release();
+    return TTAD_OK;
+}
+
+int cholesky_persistent(ttad_context* ctx, ttad_matrix* M, int prio_kind, int prec, hipStream_t s, int* info) {
+    (void)info;
+    const auto key = std::make_tuple(M->id, prio_kind, prec);
+    ttad_graph* g = nullptr;
+    auto it = ctx->chol_graphs.find(key);
+    if (it == ctx->chol_graphs.end()) {
+        const int rc = build_cholesky_graph(ctx, M->nt, M->storage, prec, prio_kind, &g);
+        if (rc != TTAD_OK) return rc;
+        ctx->chol_graphs[key] = g;
+    } else {
+        g = it->second;
+    }
+    int rc = ttad_graph_bind(g, 0, M);
+    if (rc != TTAD_OK) return rc;
+    rc = ttad_graph_launch(g, 0, 0, s);
+    if (rc != TTAD_OK) return rc;
+    return ttad_graph_wait(g, 120.0, nullptr);
+}
+
+}  // namespace ttad
+
+extern "C" int ttad_cholesky_dag(int num_blocks, int prio_kind, int* n_tasks, int* n_edges, ttad_task_desc* tasks,
+                                 int32_t* indegree, double* priority, int32_t* out_ptr, int32_t* out_idx) {
+    using namespace ttad;
+    if (num_blocks <= 0 || num_blocks > 4096 || prio_kind < 0 || prio_kind > 3 || !n_tasks || !n_edges)
+        return set_error(TTAD_ERR_INVALID, "cholesky_dag: bad args");
+    Threadpool* tp = nullptr;
+    int rc = build_cholesky_graph(nullptr, num_blocks, TTAD_F64, TTAD_F64, prio_kind, nullptr, &tp);
+    if (rc != TTAD_OK) return rc;
+    std::unique_ptr<Threadpool> hold(tp);
+    const int n = tp->n_tasks(), e = (int)tp->out_idx().size();
+    const bool fill = tasks && indegree && out_ptr && (e == 0 || out_idx);
+    if (fill) {
+        if (*n_tasks < n || *n_edges < e) return set_error(TTAD_ERR_INVALID, "cholesky_dag: arrays too small");
+        for (int t = 0; t < n; t++) {
+            tasks[t] = tp->task_descs()[t];
+            indegree[t] = tp->indegrees()[t];
+            if (priority) priority[t] = tp->priorities()[t];
+        }
+        for (int t = 0; t <= n; t++) out_ptr[t] = tp->out_ptr()[t];
+        for (int q = 0; q < e; q++) out_idx[q] = tp->out_idx()[q];
+    }
+    *n_tasks = n;
+    *n_edges = e;
+    return TTAD_OK;
+}

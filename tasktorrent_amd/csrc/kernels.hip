@@ -1,0 +1,385 @@
+// HIP kernels of the tiled Cholesky library (gfx950).
+#include "kernels.hpp"
+#include "executor.hpp"
+
+namespace ttad {
+
+// ---------------------------------------------------------------------------
+// Host-sequenced executor: one block per work item of a batch of tasks.
+// prefix[t] = first global item of task t (exclusive scan of n_items).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NTHREADS) void k_items(const DevMat* mats, const DevTask* tasks, const int32_t* prefix,
+                                                    int n_tasks) {
+    extern __shared__ __align__(16) char lds[];
+    const int g = blockIdx.x;
+    int lo = 0, hi = n_tasks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    const DevTask t = tasks[lo];
+    exec_item(t, g - prefix[lo], mats, lds);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent DAG executor. Every workgroup loops:
+//   wave 0 scans the priority buckets (one bucket per lane), claims the head
+//   entry of the highest-priority non-empty bucket with a CAS, the workgroup runs
+//   that work item, then thread 0 decrements the task's remaining-items counter;
+//   the last item of a task decrements its successors' dependency counters and
+//   pushes every item of a successor that reaches zero into its bucket.
+// Memory ordering: item outputs are released (agent scope) before the counter
+// RMWs; queue entries are published with release stores and read with acquire
+// loads, so a consumer on another XCD sees the producer's tiles.
+// The loop ends when all items are done or the abort flag is raised (watchdog:
+// no work found for spin_limit ticks; queue overflow).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void push_task(const DevGraph& g, int s) {
+    const DevTask t = g.tasks[s];
+    const int b = t.bucket;
+    const int pos = __hip_atomic_fetch_add(g.q_tail + b, t.n_items, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int cap = g.q_base[b + 1] - g.q_base[b];
+    if (pos + t.n_items > cap) {
+        __hip_atomic_store(g.abort_flag, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    for (int r = 0; r < t.n_items; r++)
+        __hip_atomic_store(g.q_items + g.q_base[b] + pos + r, ((int64_t)s << 20) | r, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(NTHREADS) void k_persistent(DevGraph g) {
+    extern __shared__ __align__(16) char lds[];
+    __shared__ int64_t s_item;
+    const int tid = threadIdx.x;
+    for (;;) {
+        if (tid < 64) {
+            int64_t got = -1;
+            const int64_t t_start = (int64_t)wall_clock64();
+            for (;;) {
+                int fb = -1, fh = 0;
+                for (int base = 0; base < g.n_buckets; base += 64) {
+                    const int b = base + tid;
+                    int h = 0;
+                    bool ne = false;
+                    if (b < g.n_buckets) {
+                        h = ld_relaxed(g.q_head + b);
+                        ne = h < ld_relaxed(g.q_tail + b);
+                    }
+                    const unsigned long long mask = __ballot(ne);
+                    if (mask) {
+                        const int l = __ffsll((long long)mask) - 1;
+                        fb = base + l;
+                        fh = __shfl(h, l);
+                        break;
+                    }
+                }
+                if (fb >= 0) {
+                    int64_t e = -1;
+                    if (tid == 0) {
+                        int h = fh;
+                        if (__hip_atomic_compare_exchange_strong(g.q_head + fb, &h, fh + 1, __ATOMIC_RELAXED,
+                                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                            const int64_t* slot = g.q_items + g.q_base[fb] + fh;
+                            while ((e = ld_acquire64(slot)) < 0) {
+                                if ((int64_t)wall_clock64() - t_start > g.spin_limit) {
+                                    __hip_atomic_store(g.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                    e = -2;
+                                    break;
+                                }
+                                __builtin_amdgcn_s_sleep(1);
+                            }
+                        }
+                    }
+                    e = __shfl(e, 0);
+                    if (e >= 0 || e == -2) {
+                        got = e;
+                        break;
+                    }
+                    continue;  // lost the race for that slot: rescan
+                }
+                if (ld_relaxed(g.done_items) >= g.total_items || ld_relaxed(g.abort_flag) != 0) {
+                    got = -2;
+                    break;
+                }
+                if ((int64_t)wall_clock64() - t_start > g.spin_limit) {
+                    if (tid == 0) __hip_atomic_store(g.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    got = -2;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            if (tid == 0) s_item = got;
+        }
+        __syncthreads();
+        const int64_t e = s_item;
+        if (e < 0) break;
+        __atomic_thread_fence(__ATOMIC_ACQUIRE);
+        const int task = (int)(e >> 20), item = (int)(e & 0xFFFFF);
+        const DevTask t = g.tasks[task];
+        uint64_t t0 = 0;
+        if (g.trace && tid == 0) t0 = wall_clock64();
+        exec_item(t, item, g.mats, lds);
+        __syncthreads();
+        if (tid == 0) {
+            __atomic_thread_fence(__ATOMIC_RELEASE);
+            if (g.trace) {
+                const int slot = __hip_atomic_fetch_add(g.trace_cursor, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (slot < g.total_items) {
+                    uint64_t* rec = g.trace + 4 * (long)slot;
+                    rec[0] = (uint64_t)e;
+                    rec[1] = blockIdx.x;
+                    rec[2] = t0;
+                    rec[3] = wall_clock64();
+                }
+            }
+            const int left =
+                __hip_atomic_fetch_add(g.items_left + task, -1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) - 1;
+            if (left == 0) {
+                for (int q = g.out_ptr[task]; q < g.out_ptr[task + 1]; q++) {
+                    const int s = g.out_idx[q];
+                    const int d =
+                        __hip_atomic_fetch_add(g.deps + s, -1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) - 1;
+                    if (d == 0) push_task(g, s);
+                }
+            }
+            __hip_atomic_fetch_add(g.done_items, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// Restores the counters and queues of a graph so that it can run again.
+__global__ void k_graph_reset_a(DevGraph g, int n_slots) {
+    const int stride = gridDim.x * blockDim.x;
+    for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < n_slots; x += stride) g.q_items[x] = -1;
+    for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < g.n_tasks; x += stride) {
+        g.deps[x] = g.deps0[x];
+        g.items_left[x] = g.tasks[x].n_items;
+    }
+    for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < g.n_buckets; x += stride) {
+        g.q_head[x] = 0;
+        g.q_tail[x] = g.tail0[x];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *g.done_items = 0;
+        *g.abort_flag = 0;
+        if (g.trace_cursor) *g.trace_cursor = 0;
+    }
+}
+__global__ void k_graph_reset_b(DevGraph g) {
+    const int stride = gridDim.x * blockDim.x;
+    for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < g.n_init; x += stride) {
+        const int64_t v = g.init_items[x];
+        const int64_t slot = v >> 40;
+        g.q_items[slot] = v & ((1LL << 40) - 1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Raw tile kernels behind the BLAS-shaped C-ABI entry points.
+// ---------------------------------------------------------------------------
+template <typename TS, int PREC>
+__global__ __launch_bounds__(NTHREADS) void k_raw_gemm(const TS* A, long lda, const TS* B, long ldb, TS* C, long ldc,
+                                                       int m, int n, int K, int syrk, double alpha, double beta) {
+    extern __shared__ __align__(16) char lds[];
+    int rb, cb;
+    decode_gemm_item(blockIdx.x, (m + GEMM_BM - 1) / GEMM_BM, syrk != 0, rb, cb);
+    item_gemm<TS, PREC>(A, lda, B, ldb, C, ldc, m, n, K, rb * GEMM_BM, cb * GEMM_BM, syrk != 0, alpha, beta, lds);
+}
+
+template <typename TS>
+__global__ __launch_bounds__(NTHREADS) void k_raw_potrf(TS* A, long lda, int n, double* dinv, int* info) {
+    extern __shared__ __align__(16) char lds[];
+    item_potrf<TS>(A, lda, n, dinv, info, 0, lds);
+}
+
+template <typename TS, typename TC>
+__global__ __launch_bounds__(NTHREADS) void k_raw_trsm(const TS* L, long ldl, const double* dinv, TS* X, long ldx,
+                                                       int m, int n) {
+    extern __shared__ __align__(16) char lds[];
+    item_trsm<TS, TC>(L, ldl, dinv, X, ldx, m, n, blockIdx.x * TRSM_BM, lds);
+}
+
+// Inverses of the 64x64 diagonal blocks of a lower-triangular n x n L (one block per workgroup).
+template <typename TS>
+__global__ __launch_bounds__(NTHREADS) void k_trtri_diag(const TS* L, long ldl, int n, double* dinv) {
+    __shared__ double D[DB * DP];
+    const int pb = blockIdx.x, p0 = pb * DB, pw = min(DB, n - p0);
+    for (int e = threadIdx.x; e < DB * DB; e += NTHREADS) {
+        const int i = e % DB, j = e / DB;
+        D[i + j * DP] = (i < pw && j < pw) ? (double)L[(p0 + i) + (long)(p0 + j) * ldl] : (i == j ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    lds_trtri64(D, pw);
+    double* X = dinv + (long)pb * DB * DB;
+    for (int e = threadIdx.x; e < DB * DB; e += NTHREADS) X[e] = D[(e % DB) + (e / DB) * DP];
+}
+
+// ---------------------------------------------------------------------------
+// Tile layout conversion: full column-major n x n (FP64) <-> lower tiles.
+// One block per (i, j) tile of the lower triangle, j = blockIdx.y, i = blockIdx.x.
+// ---------------------------------------------------------------------------
+template <typename TS>
+__global__ void k_pack(const double* __restrict__ src, long lds_, const DevMat* M, int nt) {
+    const int i = blockIdx.x, j = blockIdx.y;
+    if (i < j) return;
+    const DevMat& m = *M;
+    TS* t = (TS*)m.tiles[i + (long)j * nt];
+    const int r = m.bsz[i], c = m.bsz[j], ri = m.boff[i], cj = m.boff[j];
+    const long cnt = (long)r * c;
+    for (long e = threadIdx.x; e < cnt; e += blockDim.x) {
+        const int p = (int)(e % r), q = (int)(e / r);
+        t[e] = (TS)src[(ri + p) + (long)(cj + q) * lds_];
+    }
+}
+template <typename TS>
+__global__ void k_unpack(double* __restrict__ dst, long ldd, const DevMat* M, int nt, int zero_upper) {
+    const int i = blockIdx.x, j = blockIdx.y;
+    const DevMat& m = *M;
+    const int ri = m.boff[i], cj = m.boff[j], r = m.bsz[i], c = m.bsz[j];
+    const long cnt = (long)r * c;
+    if (i < j) {
+        if (zero_upper)
+            for (long e = threadIdx.x; e < cnt; e += blockDim.x)
+                dst[(ri + (int)(e % r)) + (long)(cj + (int)(e / r)) * ldd] = 0.0;
+        return;
+    }
+    const TS* t = (const TS*)m.tiles[i + (long)j * nt];
+    for (long e = threadIdx.x; e < cnt; e += blockDim.x) {
+        const int p = (int)(e % r), q = (int)(e / r);
+        if (i == j && p < q) {
+            if (zero_upper) dst[(ri + p) + (long)(cj + q) * ldd] = 0.0;
+            continue;
+        }
+        dst[(ri + p) + (long)(cj + q) * ldd] = (double)t[e];
+    }
+}
+// Tile-wise copy with conversion between two matrices of the same partition.
+template <typename TD, typename TSRC>
+__global__ void k_copy_tiles(const DevMat* S, const DevMat* D, int nt) {
+    const int i = blockIdx.x, j = blockIdx.y;
+    if (i < j) return;
+    const TSRC* s = (const TSRC*)S->tiles[i + (long)j * nt];
+    TD* d = (TD*)D->tiles[i + (long)j * nt];
+    const long cnt = (long)S->bsz[i] * S->bsz[j];
+    for (long e = threadIdx.x; e < cnt; e += blockDim.x) d[e] = (TD)s[e];
+}
+
+// ---------------------------------------------------------------------------
+// Launch wrappers (host side, C++ linkage; called by the C-ABI layer).
+// ---------------------------------------------------------------------------
+static hipError_t set_lds(const void* f, int bytes) {
+    return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+hipError_t kernels_init() {
+    hipError_t e = hipSuccess;
+#define TTAD_SETLDS(fn)                                          \
+    do {                                                         \
+        e = set_lds((const void*)(fn), LDS_ITEM_MAX);            \
+        if (e != hipSuccess) return e;                           \
+    } while (0)
+    TTAD_SETLDS(k_items);
+    TTAD_SETLDS(k_persistent);
+    TTAD_SETLDS((k_raw_gemm<double, P_F64>));
+    TTAD_SETLDS((k_raw_gemm<float, P_F32>));
+    TTAD_SETLDS((k_raw_gemm<float, P_BF16>));
+    TTAD_SETLDS((k_raw_potrf<double>));
+    TTAD_SETLDS((k_raw_potrf<float>));
+    TTAD_SETLDS((k_raw_trsm<double, double>));
+    TTAD_SETLDS((k_raw_trsm<float, float>));
+#undef TTAD_SETLDS
+    return e;
+}
+
+int persistent_blocks_per_cu() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_persistent, NTHREADS, LDS_ITEM_MAX) !=
+        hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
+}
+
+void launch_items(const DevMat* mats, const DevTask* tasks, const int32_t* prefix, int n_tasks, int n_items,
+                  hipStream_t s) {
+    if (n_items <= 0) return;
+    hipLaunchKernelGGL(k_items, dim3(n_items), dim3(NTHREADS), LDS_ITEM_MAX, s, mats, tasks, prefix, n_tasks);
+}
+
+void launch_graph_reset(const DevGraph& g, int n_slots, hipStream_t s) {
+    hipLaunchKernelGGL(k_graph_reset_a, dim3(256), dim3(256), 0, s, g, n_slots);
+    hipLaunchKernelGGL(k_graph_reset_b, dim3(64), dim3(256), 0, s, g);
+}
+
+void launch_persistent(const DevGraph& g, int n_workers, hipStream_t s) {
+    hipLaunchKernelGGL(k_persistent, dim3(n_workers), dim3(NTHREADS), LDS_ITEM_MAX, s, g);
+}
+
+void launch_raw_gemm(int prec, const void* A, long lda, const void* B, long ldb, void* C, long ldc, int m, int n,
+                     int K, int syrk, double alpha, double beta, hipStream_t s) {
+    const int tr = (m + GEMM_BM - 1) / GEMM_BM, tc = (n + GEMM_BM - 1) / GEMM_BM;
+    const int items = syrk ? tr * (tr + 1) / 2 : tr * tc;
+    if (items <= 0) return;
+    if (prec == P_F64)
+        hipLaunchKernelGGL((k_raw_gemm<double, P_F64>), dim3(items), dim3(NTHREADS), LDS_ITEM_MAX, s,
+                           (const double*)A, lda, (const double*)B, ldb, (double*)C, ldc, m, n, K, syrk, alpha, beta);
+    else if (prec == P_F32)
+        hipLaunchKernelGGL((k_raw_gemm<float, P_F32>), dim3(items), dim3(NTHREADS), LDS_ITEM_MAX, s, (const float*)A,
+                           lda, (const float*)B, ldb, (float*)C, ldc, m, n, K, syrk, alpha, beta);
+    else
+        hipLaunchKernelGGL((k_raw_gemm<float, P_BF16>), dim3(items), dim3(NTHREADS), LDS_ITEM_MAX, s,
+                           (const float*)A, lda, (const float*)B, ldb, (float*)C, ldc, m, n, K, syrk, alpha, beta);
+}
+
+void launch_raw_potrf(int prec, void* A, long lda, int n, double* dinv, int* info, hipStream_t s) {
+    if (prec == P_F64)
+        hipLaunchKernelGGL((k_raw_potrf<double>), dim3(1), dim3(NTHREADS), LDS_ITEM_MAX, s, (double*)A, lda, n, dinv,
+                           info);
+    else
+        hipLaunchKernelGGL((k_raw_potrf<float>), dim3(1), dim3(NTHREADS), LDS_ITEM_MAX, s, (float*)A, lda, n, dinv,
+                           info);
+}
+
+void launch_trtri_diag(int prec, const void* L, long ldl, int n, double* dinv, hipStream_t s) {
+    const int nb = (n + DB - 1) / DB;
+    if (prec == P_F64)
+        hipLaunchKernelGGL((k_trtri_diag<double>), dim3(nb), dim3(NTHREADS), 0, s, (const double*)L, ldl, n, dinv);
+    else
+        hipLaunchKernelGGL((k_trtri_diag<float>), dim3(nb), dim3(NTHREADS), 0, s, (const float*)L, ldl, n, dinv);
+}
+
+void launch_raw_trsm(int prec, const void* L, long ldl, const double* dinv, void* X, long ldx, int m, int n,
+                     hipStream_t s) {
+    const int items = (m + TRSM_BM - 1) / TRSM_BM;
+    if (items <= 0) return;
+    if (prec == P_F64)
+        hipLaunchKernelGGL((k_raw_trsm<double, double>), dim3(items), dim3(NTHREADS), LDS_ITEM_MAX, s,
+                           (const double*)L, ldl, dinv, (double*)X, ldx, m, n);
+    else
+        hipLaunchKernelGGL((k_raw_trsm<float, float>), dim3(items), dim3(NTHREADS), LDS_ITEM_MAX, s, (const float*)L,
+                           ldl, dinv, (float*)X, ldx, m, n);
+}
+
+void launch_pack(int storage, const double* src, long lda, const DevMat* M, int nt, hipStream_t s) {
+    if (storage == P_F64) hipLaunchKernelGGL((k_pack<double>), dim3(nt, nt), dim3(256), 0, s, src, lda, M, nt);
+    else hipLaunchKernelGGL((k_pack<float>), dim3(nt, nt), dim3(256), 0, s, src, lda, M, nt);
+}
+
+void launch_unpack(int storage, double* dst, long ldd, const DevMat* M, int nt, int zero_upper, hipStream_t s) {
+    if (storage == P_F64)
+        hipLaunchKernelGGL((k_unpack<double>), dim3(nt, nt), dim3(256), 0, s, dst, ldd, M, nt, zero_upper);
+    else hipLaunchKernelGGL((k_unpack<float>), dim3(nt, nt), dim3(256), 0, s, dst, ldd, M, nt, zero_upper);
+}
+
+void launch_copy_tiles(int dst_storage, int src_storage, const DevMat* S, const DevMat* D, int nt, hipStream_t s) {
+    if (dst_storage == P_F32 && src_storage == P_F64)
+        hipLaunchKernelGGL((k_copy_tiles<float, double>), dim3(nt, nt), dim3(256), 0, s, S, D, nt);
+    else if (dst_storage == P_F64 && src_storage == P_F32)
+        hipLaunchKernelGGL((k_copy_tiles<double, float>), dim3(nt, nt), dim3(256), 0, s, S, D, nt);
+    else if (dst_storage == P_F64)
+        hipLaunchKernelGGL((k_copy_tiles<double, double>), dim3(nt, nt), dim3(256), 0, s, S, D, nt);
+    else hipLaunchKernelGGL((k_copy_tiles<float, float>), dim3(nt, nt), dim3(256), 0, s, S, D, nt);
+}
+
+}  // namespace ttad

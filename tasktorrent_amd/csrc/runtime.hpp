@@ -1,0 +1,100 @@
+// Host-side runtime objects behind the C-ABI handles.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/tasktorrent_amd.h"
+#include "executor.hpp"
+#include "kernels.hpp"
+
+namespace ttad {
+
+int set_error(int code, const char* fmt, ...);
+
+#define TTAD_HIP(call)                                                                                   \
+    do {                                                                                                 \
+        hipError_t _e = (call);                                                                          \
+        if (_e != hipSuccess) return ::ttad::set_error(TTAD_ERR_DEVICE, "%s: %s", #call, hipGetErrorString(_e)); \
+    } while (0)
+
+// A device buffer owned by a handle.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t b) {
+        release();
+        if (b == 0) b = 16;
+        hipError_t e = hipMalloc(&p, b);
+        if (e == hipSuccess) bytes = b;
+        else p = nullptr;
+        return e;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct StreamPlan;  // cached host-sequenced schedule
+
+}  // namespace ttad
+
+struct ttad_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n_cu = 0;
+    long long hbm_bytes = 0;
+    std::string name;
+    int workers_per_cu = 1;
+    ttad::DevBuf scratch_dinv;  // inverse diagonal blocks for the raw tile entry points
+    ttad::DevBuf scratch_info;  // one int
+    // host-sequenced schedules, keyed by (matrix id, prec)
+    std::map<std::pair<long long, int>, std::shared_ptr<ttad::StreamPlan>> stream_plans;
+    // persistent graphs for ttad_potrf_tiled, keyed by (matrix id, prio kind, prec)
+    std::map<std::tuple<long long, int, int>, ttad_graph*> chol_graphs;
+    long long next_matrix_id = 1;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;  // bracket the last ttad_potrf_tiled on its stream
+    bool ev_valid = false;
+};
+
+struct ttad_matrix {
+    ttad_context* ctx = nullptr;
+    long long id = 0;
+    int nt = 0, n = 0, storage = TTAD_F64;
+    std::vector<int> bsz, boff;
+    std::vector<size_t> tile_off;  // element offset of tile (i,j) (i >= j), index i + j*nt
+    ttad::DevBuf tiles, dinv, tab_bsz, tab_boff, tab_tiles, tab_dinv, info, desc;
+    size_t elem = 8;
+    long long bytes = 0;
+    ttad::DevMat host_desc{};
+};
+
+struct ttad_graph {
+    ttad_context* ctx = nullptr;
+    int n_tasks = 0, n_buckets = 0, total_items = 0;
+    std::vector<ttad::DevTask> tasks;
+    std::vector<int32_t> indegree, seeds, out_ptr, out_idx, q_base, bucket_items;
+    std::vector<ttad_matrix*> mats;
+    bool seeds_dirty = true, mats_dirty = true, running = false;
+    ttad::DevBuf d_tasks, d_out_ptr, d_out_idx, d_deps0, d_tail0, d_init, d_deps, d_items_left, d_q_head, d_q_tail,
+        d_q_base, d_q_items, d_done, d_abort, d_mats, d_trace, d_trace_cursor;
+    int n_init = 0;
+    int workers = 0;
+    bool traced = false;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    hipStream_t stream = nullptr;
+    std::vector<std::string> names;  // task names for traces (optional)
+};

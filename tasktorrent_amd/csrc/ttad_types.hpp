@@ -1,0 +1,51 @@
+// Shared host/device definitions of the tile DAG executor.
+#pragma once
+#include <stdint.h>
+
+namespace ttad {
+
+// Task kinds of the tiled Cholesky PTG (miniapp/dense_cholesky/2d_cholesky.cpp:316-608).
+enum Kind : int32_t { K_POTRF = 0, K_TRSM = 1, K_SYRK = 2, K_GEMM = 3, K_ACCU = 4, K_NOP = 5 };
+// Arithmetic of a task. Tiles are stored in FP64 (P_F64) or FP32 (the mixed variants).
+enum Prec : int32_t { P_F64 = 0, P_F32 = 1, P_BF16 = 2 };
+
+constexpr int NTHREADS = 256;  // one workgroup = 4 wave64
+constexpr int GEMM_BM = 128;   // GEMM/SYRK work item = 128x128 output sub-tile
+constexpr int TRSM_BM = 64;    // TRSM work item = 64-row block of the tile
+constexpr int DB = 64;         // diagonal sub-block size whose inverse POTRF leaves for TRSM
+
+// One tiled symmetric matrix in HBM. Tile (i,j), i >= j, is a column-major
+// bsz[i] x bsz[j] array with leading dimension bsz[i] (block_sizes_lda,
+// 2d_cholesky.cpp:86-89), reached through the pointer table.
+struct DevMat {
+    int32_t nt;
+    int32_t storage;  // P_F64 or P_F32
+    const int32_t* bsz;
+    const int32_t* boff;
+    void* const* tiles;    // [nt*nt], entry i + j*nt
+    double* const* dinv;   // [nt]: inverses of the 64x64 diagonal blocks of L_jj (FP64, 64x64 each, ld 64)
+    int32_t* info;         // first non-positive pivot, 1-based global column; 0 = none
+    int32_t* accbuf_ok;    // unused slot kept for layout stability
+};
+
+// 32-byte device task record.
+struct DevTask {
+    int32_t kind;
+    int32_t prec;
+    int32_t mat;
+    int32_t n_items;
+    int32_t i, j, k;
+    int32_t bucket;
+};
+
+// Work items per task; host and device must agree.
+__host__ __device__ inline int items_of(int kind, int bi, int bj) {
+    if (kind == K_POTRF) return 1;
+    if (kind == K_TRSM) return (bi + TRSM_BM - 1) / TRSM_BM;
+    const int tr = (bi + GEMM_BM - 1) / GEMM_BM;
+    if (kind == K_SYRK) return tr * (tr + 1) / 2;
+    if (kind == K_GEMM) return tr * ((bj + GEMM_BM - 1) / GEMM_BM);
+    return 1;
+}
+
+}  // namespace ttad

@@ -1,0 +1,77 @@
+"""CPU checks of the C-ABI library (no GPU calls):
+- it loads and exports every function include/tasktorrent_amd.h declares;
+- the Cholesky DAG compiled through the Taskflow drop-in equals, edge for edge,
+  the dependencies the oracle's restated Taskflow/Threadpool records while
+  running 2d_cholesky.cpp's PTG (its --depslog edges, 2d_cholesky.cpp:325-329,
+  414-418, 509-528), with the same indegrees and PrioKind priorities.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import tasktorrent_amd as tt
+from tasktorrent_amd import _capi
+
+
+def test_library_exports_header_symbols():
+    L = tt.lib()
+    declared = _capi.declared_symbols()
+    assert len(declared) >= 30
+    missing = [s for s in declared if not hasattr(L, s)]
+    assert not missing, missing
+    assert L.ttad_version() >= 1
+
+
+def test_bindings_cover_header():
+    declared = set(_capi.declared_symbols())
+    assert declared <= set(_capi._SIGS), declared - set(_capi._SIGS)
+
+
+def test_invalid_args_return_codes():
+    L = tt.lib()
+    assert L.ttad_init(0, None) == _capi.TTAD_ERR_INVALID
+    assert b"NULL" in L.ttad_last_error()
+    n, e = ctypes.c_int(), ctypes.c_int()
+    assert L.ttad_cholesky_dag(0, 2, ctypes.byref(n), ctypes.byref(e), None, None, None, None, None) != 0
+    assert L.ttad_cholesky_dag(4, 7, ctypes.byref(n), ctypes.byref(e), None, None, None, None, None) != 0
+
+
+@pytest.mark.parametrize("nb", [1, 2, 5, 8])
+def test_dag_counts(nb):
+    tasks, indeg, prio, out_ptr, out_idx = tt.cholesky_dag(nb, 2)
+    kinds = [t[0] for t in tasks]
+    assert kinds.count(tt.TTAD_POTRF) == nb
+    assert kinds.count(tt.TTAD_TRSM) == nb * (nb - 1) // 2
+    assert kinds.count(tt.TTAD_SYRK) == nb * (nb - 1) // 2
+    assert kinds.count(tt.TTAD_GEMM) == sum((nb - k - 1) * (nb - k - 2) // 2 for k in range(nb))
+    # every task receives exactly its indegree: edges + the single host seed of POTRF_0
+    incoming = np.bincount(out_idx, minlength=len(tasks))
+    seed = np.array([1 if tt.task_name(t) == "POTRF_0" else 0 for t in tasks])
+    assert np.array_equal(incoming + seed, indeg)
+
+
+@pytest.mark.parametrize("nb,prio_kind", [(4, 2), (6, 0), (7, 3), (5, 1)])
+def test_dag_equals_oracle_deps(oracle, tmp_path, nb, prio_kind):
+    tasks, indeg, prio, out_ptr, out_idx = tt.cholesky_dag(nb, prio_kind)
+    names = [tt.task_name(t) for t in tasks]
+    ours = {(names[u], names[v]) for u in range(len(tasks)) for v in out_idx[out_ptr[u]:out_ptr[u + 1]]}
+    A = oracle.spd_bbt(nb * 8, 1)
+    deps = tmp_path / "deps.csv"
+    oracle.cholesky(A, 8, n_threads=3, prio_kind=prio_kind, deps_path=str(deps))
+    theirs = {tuple(line.split(",")) for line in deps.read_text().splitlines() if line}
+    assert ours == theirs
+    assert len(out_idx) == len(theirs)  # no duplicated edge
+
+
+def test_dag_priorities_match_prio_kind():
+    nb = 6
+    for kind in range(4):
+        tasks, indeg, prio, _, _ = tt.cholesky_dag(nb, kind)
+        for t, p in zip(tasks, prio):
+            k_, _, _, i, j, k = t
+            if kind == 2:
+                exp = {tt.TTAD_POTRF: 9 * (nb - j) - 1, tt.TTAD_TRSM: 9 * (nb - j) - 2}.get(k_, 9 * nb - 9 * j - 2)
+                assert p == exp
+            if kind == 0:
+                assert p == {tt.TTAD_POTRF: 3.0, tt.TTAD_TRSM: 2.0}.get(k_, 1.0)
