@@ -1,19 +1,26 @@
-"""Benchmark of the MI355X tiled Cholesky (one GPU).
+"""Benchmark of the MI355X tiled Cholesky solver (one GPU).
 
-Workload (default): n = 32768, tile b = 512, A = D (B B^T/n + delta I) D with B iid
-N(0,1) (torch generator, seed 2022), delta = 1e-2, D_ii = 10^(2U), U ~ U(0,1)
-(seed 2023): the ill-conditioned mixed-precision configuration. A step is one
-tiled Cholesky factorization of A on the device (the refreshed working copy of
-A is restored from a resident FP64 copy inside the step), through the
-persistent DAG executor.
+Workload (default; the mixed-precision configuration the metric is quoted on):
+  n = 32768, tile b = 512, A = D (B B^T/n + 1e-2 I) D with B iid N(0,1) (torch
+  generator seed 2022), D_ii = 10^(2U), U ~ U(0,1) (seed 2023), kappa ~ 1e4-1e6;
+  16 right-hand sides iid N(0,1) (seed 2024). Inputs are resident in HBM before
+  timing starts.
 
-Prints one JSON line. `value` = F(n)/t in TFLOP/s with F(n) = n^3/3 + n^2/2 + n/6.
+A step is one time-to-solution of A X = B to normwise backward error <= 1e-12:
+  - fp64 leg: copy A's tiles into the working factor, persistent-DAG FP64 Cholesky,
+    FP64 refinement (normally zero or one step);
+  - mixed legs: cast A's tiles to FP32, persistent-DAG Cholesky with the trailing
+    update in BF16x3 / FP32 / BF16 (FP64 diagonal POTRF), FP64 refinement with
+    fallback to FP64 when refinement stalls.
+`value` = F(n) / t_step of the headline mixed leg, in TFLOP/s of FP64-equivalent
+work, F(n) = n^3/3 + n^2/2 + n/6 (so refinement time counts against it).
+
+Prints one JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -22,7 +29,8 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-FP64_PEAK = 78.6  # TFLOP/s, MI355X dense FP64 matrix
+PEAK = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}  # TFLOP/s, MI355X dense MFMA
+METRIC = "Cholesky TFLOP/s n=32768 (mixed+IR vs FP64, <=1e-12 resid); p50/p99 solve ms"
 
 
 def flops(n: int) -> float:
@@ -49,23 +57,25 @@ def make_matrix(n: int, seed_b: int, seed_d: int, delta: float, scaled: bool):
         A *= d[:, None]
         A *= d[None, :]
     torch.cuda.synchronize()
-    return A  # symmetric, so row-major == column-major
+    return A  # symmetric, so row-major storage equals column-major
 
 
 def cpu_baseline(n_sample: int, b: int, threads: int):
-    """The oracle (Taskflow+Threadpool restatement with sequential MKL per task) on the host."""
+    """The oracle (Taskflow+Threadpool restatement, sequential MKL per task) on the host cores."""
     import numpy as np
     from oracle import oracle as orc
     orc.build()
     kind = orc.set_backend("mkl") if orc.have_mkl() else orc.set_backend("textbook")
     A = orc.spd_bbt(n_sample, 2020)
+    orc.cholesky(A, b, n_threads=threads, prio_kind=2)  # warm-up, as 2d_cholesky.cpp:54-61
     times = []
     for _ in range(3):
         _, st = orc.cholesky(A, b, n_threads=threads, prio_kind=2)
         times.append(st["total"])
     t = float(np.median(times))
-    return {"value": flops(n_sample) / t / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
-            "sample": f"n={n_sample} b={b} FP64 A=BB^T/n+I, median of 3 runs, {t:.3f} s/run, backend={kind}"}
+    return {"value": round(flops(n_sample) / t / 1e12, 4), "unit": "TFLOP/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 tiled Cholesky n={n_sample} b={b}, A=BB^T/n+I, seed->join window, median of 3 "
+                      f"({t:.3f} s/run), backend={kind}"}
 
 
 def main():
@@ -75,76 +85,135 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=32768)
     ap.add_argument("--b", type=int, default=512)
-    ap.add_argument("--prio", type=int, default=2)
-    ap.add_argument("--schedule", default="persistent", choices=["persistent", "stream"])
+    ap.add_argument("--nrhs", type=int, default=16)
+    ap.add_argument("--prec", default="bf16x3", help="headline mixed precision")
+    ap.add_argument("--extra-precs", default="f32,bf16", help="other mixed legs (one timed run each)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=8192)
+    ap.add_argument("--fp64-only", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it (--gpus 1)"}))
         sys.exit(2)
 
+    import numpy as np
     import torch
     import tasktorrent_amd as tt
 
-    n, b = args.n, args.b
-    sched = tt.TTAD_SCHED_PERSISTENT if args.schedule == "persistent" else tt.TTAD_SCHED_STREAM
+    n, b, nrhs = args.n, args.b, args.nrhs
+    F = flops(n)
     ctx = tt.Context(0)
-    log(f"device {ctx.info()}")
+    log(f"device {ctx.info()}; n={n} b={b} nrhs={nrhs}")
     A = make_matrix(n, 2022, 2023, 1e-2, True)
     sizes = [b] * (n // b) + ([n % b] if n % b else [])
     A64 = tt.TiledMatrix(ctx, sizes, "f64")
     A64.set(A)
-    W = tt.TiledMatrix(ctx, sizes, "f64")
     del A
     torch.cuda.empty_cache()
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2024)
+    Bt = torch.randn((nrhs, n), dtype=torch.float64, device="cuda", generator=g)  # column-major n x nrhs
+    Xt = torch.zeros_like(Bt)
+    W64 = tt.TiledMatrix(ctx, sizes, "f64")
     ctx.synchronize()
+    torch.cuda.synchronize()
 
-    def step():
-        W.copy_from(A64)
-        info = ctx.potrf_tiled(W, sched, args.prio, "f64")
+    # ---- FP64 leg: factor + FP64 refinement ----
+    def fp64_step():
+        t0 = time.perf_counter()
+        W64.copy_from(A64)
+        info = ctx.potrf_tiled(W64, tt.TTAD_SCHED_PERSISTENT, 2, "f64")
         if info != 0:
-            raise RuntimeError(f"factorization failed, info={info}")
-        return ctx.last_elapsed()
+            raise RuntimeError(f"FP64 factorization failed, info={info}")
+        t_fac = ctx.last_elapsed()
+        st = ctx.solve_ir(A64, W64, nrhs, Bt, Xt, tol=1e-12, max_iter=10)
+        ctx.synchronize()
+        return time.perf_counter() - t0, t_fac, st
 
+    res = {}
     for w in range(args.warmup):
-        t = step()
-        log(f"warmup {w}: factor {t * 1e3:.2f} ms")
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    dev_times = []
-    t0 = time.perf_counter()
+        t, tf, st = fp64_step()
+        log(f"fp64 warmup {w}: tts {t*1e3:.1f} ms factor {tf*1e3:.1f} ms eta {st['backward_error']:.2e}")
+    tts64, fac64 = [], []
     for s in range(args.steps):
-        dev_times.append(step())
-        log(f"step {s}: factor {dev_times[-1] * 1e3:.2f} ms")
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    t_factor = sum(dev_times) / len(dev_times)
-    F = flops(n)
-    value = F / (ms_per_step * 1e-3) / 1e12
-    achieved = F / t_factor / 1e12
+        t, tf, st = fp64_step()
+        tts64.append(t)
+        fac64.append(tf)
+        log(f"fp64 step {s}: tts {t*1e3:.1f} ms factor {tf*1e3:.1f} ms ({F/tf/1e12:.2f} TF/s) "
+            f"eta {st['backward_error']:.2e} it {st['iterations']}")
+    res["f64"] = {"tts_ms": round(1e3 * float(np.median(tts64)), 3), "factor_ms": round(1e3 * float(np.median(fac64)), 3),
+                  "factor_tflops": round(F / float(np.median(fac64)) / 1e12, 3),
+                  "tts_tflops": round(F / float(np.median(tts64)) / 1e12, 3),
+                  "backward_error": st["backward_error"], "ir_steps": st["iterations"]}
+
+    # ---- mixed legs ----
+    def mixed_step(prec):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = ctx.posv_mixed(A64, prec, nrhs, Bt, Xt, tol=1e-12, max_iter=30)
+        t = time.perf_counter() - t0
+        fac = ctx.last_elapsed() if not st["fell_back"] else float("nan")
+        return t, st, fac
+
+    headline = args.prec
+    legs = [] if args.fp64_only else [headline] + [p for p in args.extra_precs.split(",") if p and p != headline]
+    timed_steps = {}
+    for prec in legs:
+        reps = args.steps if prec == headline else 1
+        for w in range(args.warmup):
+            t, st, fac = mixed_step(prec)
+            log(f"{prec} warmup {w}: tts {t*1e3:.1f} ms eta {st['backward_error']:.2e} it {st['iterations']} "
+                f"fallback {st['fell_back']} rc {st['rc']}")
+        ts, facs = [], []
+        for s in range(reps):
+            t, st, fac = mixed_step(prec)
+            ts.append(t)
+            facs.append(fac)
+            log(f"{prec} step {s}: tts {t*1e3:.1f} ms eta {st['backward_error']:.2e} it {st['iterations']} "
+                f"fallback {st['fell_back']} rc {st['rc']}")
+        timed_steps[prec] = ts
+        tmed = float(np.median(ts))
+        res[prec] = {"tts_ms": round(1e3 * tmed, 3), "tts_tflops": round(F / tmed / 1e12, 3),
+                     "backward_error": st["backward_error"], "ir_steps": st["iterations"],
+                     "fell_back": bool(st["fell_back"]), "rc": st["rc"],
+                     "speedup_vs_f64": round(res["f64"]["tts_ms"] / (1e3 * tmed), 3)}
+        if not st["fell_back"]:
+            res[prec]["last_factor_ms"] = round(1e3 * st["factor_s"], 3)
+
+    if legs and res[headline]["rc"] == 0:
+        ts = timed_steps[headline]
+        value = F / float(np.median(ts)) / 1e12
+        dtype = headline
+        steps_ms = float(np.mean(ts)) * 1e3
+        workload = f"mixed {headline}+IR time-to-solution"
+    else:
+        value = res["f64"]["tts_tflops"]
+        dtype = "f64"
+        steps_ms = float(np.mean(tts64)) * 1e3
+        workload = "FP64 time-to-solution"
+
+    t_fac64 = float(np.median(fac64))
     out = {
-        "metric": "Cholesky TFLOP/s n=32768 (mixed+IR vs FP64, <=1e-12 resid); p50/p99 solve ms",
+        "metric": METRIC,
         "value": round(value, 3),
         "unit": "TFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(steps_ms, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": dtype,
         "data": "synthetic",
-        "config": {"workload": f"FP64 tiled Cholesky n={n} b={b} A=D(BB^T/n+1e-2 I)D, {args.schedule} DAG",
-                   "n": n, "tile": b, "prio_kind": args.prio, "schedule": args.schedule},
-        "factor_ms_device": round(t_factor * 1e3, 3),
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_PEAK, 4), "traffic": None,
-                     "kernel": "k_persistent (whole factorization, one launch)"},
+        "config": {"workload": f"{workload}: Cholesky + FP64 refinement to eta<=1e-12, n={n} b={b} nrhs={nrhs}, "
+                               f"A=D(BB^T/n+1e-2 I)D",
+                   "n": n, "tile": b, "nrhs": nrhs, "prio_kind": 2, "schedule": "persistent"},
+        "legs": res,
+        "roofline": {"bound": "mfma", "achieved": round(F / t_fac64 / 1e12, 3), "peak": PEAK["f64"],
+                     "unit": "TFLOP/s", "frac": round(F / t_fac64 / 1e12 / PEAK["f64"], 4), "traffic": None,
+                     "kernel": "k_persistent FP64 factorization (one launch, F(n) flops / event-timed duration)"},
     }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

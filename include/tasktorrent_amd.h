@@ -38,6 +38,7 @@ extern "C" {
 #define TTAD_F64 0   /* FP64 tiles, FP64 MFMA */
 #define TTAD_F32 1   /* FP32 tiles, FP32 MFMA */
 #define TTAD_BF16 2  /* FP32 tiles, BF16 MFMA inputs with FP32 accumulation */
+#define TTAD_BF16X3 3 /* FP32 tiles, operands split hi+lo into two BF16s, 3 MFMAs, FP32 accumulation */
 
 /* ---- task kinds of the Cholesky PTG (2d_cholesky.cpp:316-608) ------------ */
 #define TTAD_POTRF 0
@@ -90,6 +91,10 @@ int ttad_sgemm(ttad_context* ctx, int m, int n, int k, float alpha, const float*
  * (BF16 trailing-update variant; syrk != 0 updates the lower triangle only). */
 int ttad_gemm_bf16(ttad_context* ctx, int m, int n, int k, float alpha, const float* A, int lda, const float* B,
                    int ldb, float beta, float* C, int ldc, int syrk, void* stream);
+/* Same with each operand split hi+lo into two BF16s and three MFMAs per product
+ * (about 16 significant bits per operand; BF16X3 trailing-update variant). */
+int ttad_gemm_bf16x3(ttad_context* ctx, int m, int n, int k, float alpha, const float* A, int lda, const float* B,
+                     int ldb, float beta, float* C, int ldc, int syrk, void* stream);
 
 /* ---- tiled symmetric matrices in HBM ------------------------------------------ */
 /* Replaces the per-tile storage `blocks[ii + jj*num_blocks]` of 2d_cholesky.cpp:134-147
@@ -110,7 +115,7 @@ int ttad_matrix_tile(const ttad_matrix* M, int i, int j, void** ptr, int* ld);
 /* ---- the tiled Cholesky --------------------------------------------------- */
 /* Replaces cholesky() of 2d_cholesky.cpp:41-690 on one rank, non-accumulate mode:
  * the whole factorization of M in place (lower tiles become L).
- * prec: TTAD_F64 (M FP64) or TTAD_F32 / TTAD_BF16 for the trailing update (M FP32;
+ * prec: TTAD_F64 (M FP64) or TTAD_F32 / TTAD_BF16 / TTAD_BF16X3 for the trailing update (M FP32;
  * diagonal POTRF in FP64). prio_kind 0..3 as PrioKind (:39, :157-204).
  * Synchronous; *info as ttad_dpotrf. */
 int ttad_potrf_tiled(ttad_context* ctx, ttad_matrix* M, int schedule, int prio_kind, int prec, void* stream,
@@ -124,7 +129,7 @@ int ttad_last_elapsed(ttad_context* ctx, double* seconds);
 /* ---- device task graphs: the Taskflow<K>/Threadpool drop-in ------------------- */
 typedef struct {
     int32_t kind;  /* TTAD_POTRF .. TTAD_GEMM */
-    int32_t prec;  /* TTAD_F64 / TTAD_F32 / TTAD_BF16 */
+    int32_t prec;  /* TTAD_F64 / TTAD_F32 / TTAD_BF16 / TTAD_BF16X3 */
     int32_t mat;   /* matrix slot bound with ttad_graph_bind */
     int32_t i, j, k;
 } ttad_task_desc;
@@ -158,6 +163,43 @@ int ttad_graph_wait(ttad_graph* g, double timeout_s, ttad_graph_stats* stats);
  * (util.cpp:117-127): "<prefix><worker>>run>NAME,start,end" in seconds. */
 int ttad_graph_write_trace(ttad_graph* g, const char* path, const char* prefix);
 int ttad_graph_destroy(ttad_graph* g);
+
+/* ---- solve phase ------------------------------------------------------------- */
+typedef struct {
+    int iterations;          /* refinement steps taken (all attempts) */
+    int fell_back;           /* 1 if the FP64 factorization was used after the low-precision one failed */
+    double backward_error;   /* max over columns of ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) */
+    double factor_s;         /* host wall time of the (first) factorization, seconds */
+    double solve_s;          /* host wall time of everything after it (refinement, fallback) */
+    double total_s;          /* time to solution */
+    int info;                /* factorization info of the factor used */
+    int prec_used;           /* TTAD_F64 / TTAD_F32 / TTAD_BF16 of the factor used */
+} ttad_solve_stats;
+
+/* X = (L L^T)^{-1} B with a factor left by ttad_potrf_tiled (FP64 or FP32 tiles; an
+ * FP32 factor solves in FP32). B, X: device FP64 n x nrhs, column-major. Tile sizes
+ * must be multiples of 64 except the last. Replaces the check solve
+ * L.solveInPlace(b); L.transpose().solveInPlace(b) of 2d_cholesky.cpp:676-681. */
+int ttad_solve(ttad_context* ctx, ttad_matrix* L, int nrhs, const double* B, long long ldb, double* X, long long ldx,
+               void* stream);
+/* R = B - A X in FP64 (A: FP64 tiles, symmetric from its lower triangle) and the
+ * normwise backward error max_j ||r_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf)
+ * (the b = A*x of 2d_cholesky.cpp:677 and its error check :682-687). */
+int ttad_residual(ttad_context* ctx, ttad_matrix* A, int nrhs, const double* B, long long ldb, const double* X,
+                  long long ldx, double* R, long long ldr, double* eta, void* stream);
+/* Iterative refinement in FP64 with the factor L of A (any precision): X from 0,
+ * r = B - A X, X += (L L^T)^{-1} r until eta <= tol or max_iter steps or a stall
+ * (eta not halving twice). TTAD_ERR_IR_STALLED if tol was not reached.
+ * No reference counterpart (the reference is FP64 only). */
+int ttad_solve_ir(ttad_context* ctx, ttad_matrix* A, ttad_matrix* L, int nrhs, const double* B, long long ldb,
+                  double* X, long long ldx, double tol, int max_iter, ttad_solve_stats* stats, void* stream);
+/* Mixed-precision solver: factors a working copy of A with the trailing update in
+ * prec (TTAD_F32 / TTAD_BF16 / TTAD_BF16X3; FP64 diagonal POTRF), refines in FP64 to eta <= tol,
+ * and falls back to the FP64 factorization when the low-precision factor is not
+ * positive definite or refinement stalls. The factor stays cached on the device
+ * for further right-hand sides of the same A. No reference counterpart. */
+int ttad_posv_mixed(ttad_context* ctx, ttad_matrix* A, int prec, int nrhs, const double* B, long long ldb, double* X,
+                    long long ldx, double tol, int max_iter, ttad_solve_stats* stats, void* stream);
 
 /* The Cholesky PTG of 2d_cholesky.cpp:281-614 (one rank, non-accumulate mode),
  * declared through the Taskflow drop-in and compiled on the host without a

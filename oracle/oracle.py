@@ -185,15 +185,17 @@ def spd_scaled(n: int, seed_b: int, seed_d: int, delta: float = 1e-2, dmax: floa
     return np.asfortranarray(M * d[:, None] * d[None, :])
 
 
-def int_lower_factor(n: int, seed: int) -> np.ndarray:
-    """Integer lower-triangular L with power-of-two diagonal and entries in [-2,2].
+def int_lower_factor(n: int, seed: int, offmax: int = 2, diag_exp=(0, 3)) -> np.ndarray:
+    """Integer lower-triangular L with power-of-two diagonal and entries in [-offmax, offmax].
 
-    For A = L L^T every step of a Cholesky factorization is exact in FP64 as long
-    as the magnitudes stay below 2^53, so the computed factor must equal L bitwise.
+    For A = L L^T every step of a substitution-based Cholesky (LAPACK order) is exact
+    in FP64 as long as the magnitudes stay below 2^53, so the computed factor must
+    equal L bitwise. With the defaults L is extremely ill-conditioned (its inverse
+    grows like 3^n); offmax=1, diag_exp=(3, 4) gives a well-conditioned variant.
     """
     rng = np.random.default_rng(seed)
-    L = np.tril(rng.integers(-2, 3, size=(n, n)).astype(np.float64), -1)
-    L[np.diag_indices(n)] = 2.0 ** rng.integers(0, 3, size=n)
+    L = np.tril(rng.integers(-offmax, offmax + 1, size=(n, n)).astype(np.float64), -1)
+    L[np.diag_indices(n)] = 2.0 ** rng.integers(diag_exp[0], diag_exp[1], size=n)
     return np.asfortranarray(L)
 
 

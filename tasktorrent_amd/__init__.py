@@ -15,16 +15,16 @@ from pathlib import Path
 import numpy as np
 
 from . import _capi
-from ._capi import (TTAD_BF16, TTAD_F32, TTAD_F64, TTAD_GEMM, TTAD_POTRF, TTAD_SCHED_PERSISTENT,
+from ._capi import (TTAD_BF16, TTAD_BF16X3, TTAD_F32, TTAD_F64, TTAD_GEMM, TTAD_POTRF, TTAD_SCHED_PERSISTENT,
                     TTAD_SCHED_STREAM, TTAD_SYRK, TTAD_TRSM, TtadError, lib, lib_path)
 
 __all__ = [
     "Context", "TiledMatrix", "Graph", "TtadError", "lib", "lib_path",
-    "TTAD_F64", "TTAD_F32", "TTAD_BF16", "TTAD_POTRF", "TTAD_TRSM", "TTAD_SYRK", "TTAD_GEMM",
+    "TTAD_F64", "TTAD_F32", "TTAD_BF16", "TTAD_BF16X3", "TTAD_POTRF", "TTAD_TRSM", "TTAD_SYRK", "TTAD_GEMM",
     "TTAD_SCHED_STREAM", "TTAD_SCHED_PERSISTENT",
 ]
 
-PREC = {"f64": TTAD_F64, "f32": TTAD_F32, "bf16": TTAD_BF16}
+PREC = {"f64": TTAD_F64, "f32": TTAD_F32, "bf16": TTAD_BF16, "bf16x3": TTAD_BF16X3}
 
 
 def _stream_of(torch_stream=None):
@@ -92,9 +92,10 @@ class Context:
         _capi.check(fn(self._h, m, n, k, alpha, _ptr(A), lda, _ptr(B), ldb, beta, _ptr(C), ldc,
                        stream or _stream_of()))
 
-    def gemm_bf16(self, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, syrk=False, stream=None):
-        _capi.check(lib().ttad_gemm_bf16(self._h, m, n, k, alpha, _ptr(A), lda, _ptr(B), ldb, beta, _ptr(C),
-                                         ldc, int(syrk), stream or _stream_of()))
+    def gemm_bf16(self, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, syrk=False, split=False, stream=None):
+        fn = lib().ttad_gemm_bf16x3 if split else lib().ttad_gemm_bf16
+        _capi.check(fn(self._h, m, n, k, alpha, _ptr(A), lda, _ptr(B), ldb, beta, _ptr(C), ldc, int(syrk),
+                       stream or _stream_of()))
 
     # ---- tiled Cholesky ----
     def potrf_tiled(self, M: "TiledMatrix", schedule=TTAD_SCHED_PERSISTENT, prio_kind: int = 2,
@@ -106,6 +107,44 @@ class Context:
             _capi.check(rc)
         return info.value
 
+
+    # ---- solve phase (device FP64 column-major n x nrhs arrays: torch (nrhs, n) tensors) ----
+    def solve(self, L: "TiledMatrix", nrhs: int, B, X, stream=None):
+        _capi.check(lib().ttad_solve(self._h, L.handle, nrhs, _ptr(B), B.shape[-1], _ptr(X), X.shape[-1], stream))
+        self.synchronize(stream)
+
+    def residual(self, A: "TiledMatrix", nrhs: int, B, X, R, stream=None) -> float:
+        eta = ctypes.c_double()
+        _capi.check(lib().ttad_residual(self._h, A.handle, nrhs, _ptr(B), B.shape[-1], _ptr(X), X.shape[-1],
+                                        _ptr(R), R.shape[-1], ctypes.byref(eta), stream))
+        self.synchronize(stream)
+        return eta.value
+
+    @staticmethod
+    def _stats(rc, st):
+        return {"rc": rc, "iterations": st.iterations, "fell_back": st.fell_back,
+                "backward_error": st.backward_error, "factor_s": st.factor_s, "solve_s": st.solve_s,
+                "total_s": st.total_s, "info": st.info, "prec_used": st.prec_used}
+
+    def solve_ir(self, A: "TiledMatrix", L: "TiledMatrix", nrhs: int, B, X, tol=1e-12, max_iter=30, stream=None):
+        st = _capi.SolveStats()
+        rc = lib().ttad_solve_ir(self._h, A.handle, L.handle, nrhs, _ptr(B), B.shape[-1], _ptr(X), X.shape[-1],
+                                 tol, max_iter, ctypes.byref(st), stream)
+        if rc not in (0, _capi.TTAD_ERR_IR_STALLED):
+            _capi.check(rc)
+        self.synchronize(stream)
+        return self._stats(rc, st)
+
+    def posv_mixed(self, A: "TiledMatrix", prec, nrhs: int, B, X, tol=1e-12, max_iter=30, stream=None):
+        """Mixed-precision factor + FP64 refinement with FP64 fallback (ttad_posv_mixed)."""
+        p = PREC[prec] if isinstance(prec, str) else prec
+        st = _capi.SolveStats()
+        rc = lib().ttad_posv_mixed(self._h, A.handle, p, nrhs, _ptr(B), B.shape[-1], _ptr(X), X.shape[-1], tol,
+                                   max_iter, ctypes.byref(st), stream)
+        if rc not in (0, _capi.TTAD_ERR_IR_STALLED, _capi.TTAD_ERR_NOT_SPD):
+            _capi.check(rc)
+        self.synchronize(stream)
+        return self._stats(rc, st)
 
     def last_elapsed(self) -> float:
         """Device seconds of the last potrf_tiled (HIP events on its stream)."""

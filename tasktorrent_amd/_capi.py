@@ -20,7 +20,7 @@ TTAD_ERR_GRAPH = 7
 TTAD_ERR_BUSY = 8
 TTAD_ERR_NOT_FOUND = 9
 
-TTAD_F64, TTAD_F32, TTAD_BF16 = 0, 1, 2
+TTAD_F64, TTAD_F32, TTAD_BF16, TTAD_BF16X3 = 0, 1, 2, 3
 TTAD_POTRF, TTAD_TRSM, TTAD_SYRK, TTAD_GEMM = 0, 1, 2, 3
 TTAD_SCHED_STREAM, TTAD_SCHED_PERSISTENT = 0, 1
 
@@ -75,6 +75,7 @@ _SIGS = {
     "ttad_dgemm": (I, [V, I, I, I, D, V, I, V, I, D, V, I, V]),
     "ttad_sgemm": (I, [V, I, I, I, F, V, I, V, I, F, V, I, V]),
     "ttad_gemm_bf16": (I, [V, I, I, I, F, V, I, V, I, F, V, I, I, V]),
+    "ttad_gemm_bf16x3": (I, [V, I, I, I, F, V, I, V, I, F, V, I, I, V]),
     "ttad_matrix_create": (I, [V, I, PI, I, ctypes.POINTER(V)]),
     "ttad_matrix_destroy": (I, [V]),
     "ttad_matrix_set": (I, [V, V, LL, I, V]),
@@ -94,6 +95,10 @@ _SIGS = {
     "ttad_graph_wait": (I, [V, D, ctypes.POINTER(GraphStats)]),
     "ttad_graph_write_trace": (I, [V, ctypes.c_char_p, ctypes.c_char_p]),
     "ttad_graph_destroy": (I, [V]),
+    "ttad_solve": (I, [V, V, I, V, LL, V, LL, V]),
+    "ttad_residual": (I, [V, V, I, V, LL, V, LL, V, LL, ctypes.POINTER(D), V]),
+    "ttad_solve_ir": (I, [V, V, V, I, V, LL, V, LL, D, I, ctypes.POINTER(SolveStats), V]),
+    "ttad_posv_mixed": (I, [V, V, I, I, V, LL, V, LL, D, I, ctypes.POINTER(SolveStats), V]),
     "ttad_cholesky_dag": (I, [I, I, PI, PI, ctypes.POINTER(TaskDesc), ctypes.POINTER(ctypes.c_int32),
                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32),
                               ctypes.POINTER(ctypes.c_int32)]),

@@ -136,6 +136,11 @@ int ttad_finalize(ttad_context* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (auto& kv : ctx->chol_graphs) ttad_graph_destroy(kv.second);
     ctx->chol_graphs.clear();
+    {
+        auto wm = ctx->work_mats;
+        ctx->work_mats.clear();
+        for (auto& kv : wm) ttad_matrix_destroy(kv.second);
+    }
     ctx->stream_plans.clear();
     (void)hipStreamDestroy(ctx->stream);
     if (ctx->ev_a) (void)hipEventDestroy(ctx->ev_a);
@@ -246,6 +251,13 @@ int ttad_gemm_bf16(ttad_context* ctx, int m, int n, int k, float alpha, const fl
                        stream);
 }
 
+int ttad_gemm_bf16x3(ttad_context* ctx, int m, int n, int k, float alpha, const float* A, int lda, const float* B,
+                     int ldb, float beta, float* C, int ldc, int syrk, void* stream) {
+    if (syrk && m != n) return set_error(TTAD_ERR_INVALID, "gemm_bf16x3: syrk needs m == n");
+    return gemm_common(ctx, P_BF16X3, m, n, k, alpha, A, lda, syrk ? A : B, syrk ? lda : ldb, beta, C, ldc, syrk,
+                       stream);
+}
+
 // ---------------------------------------------------------------------------
 // Tiled matrices
 // ---------------------------------------------------------------------------
@@ -322,6 +334,17 @@ int ttad_matrix_destroy(ttad_matrix* M) {
     if (!M) return TTAD_OK;
     ttad_context* ctx = M->ctx;
     (void)hipStreamSynchronize(ctx->stream);
+    for (auto it = ctx->work_mats.begin(); it != ctx->work_mats.end();) {
+        if (it->first.first == M->id) {  // working copies derived from M go with it
+            ttad_matrix* W = it->second;
+            it = ctx->work_mats.erase(it);
+            ttad_matrix_destroy(W);
+        } else if (it->second == M) {
+            it = ctx->work_mats.erase(it);
+        } else {
+            ++it;
+        }
+    }
     for (auto it = ctx->stream_plans.begin(); it != ctx->stream_plans.end();)
         it = (it->first.first == M->id) ? ctx->stream_plans.erase(it) : std::next(it);
     for (auto it = ctx->chol_graphs.begin(); it != ctx->chol_graphs.end();) {
@@ -338,6 +361,7 @@ int ttad_matrix_destroy(ttad_matrix* M) {
 
 int ttad_matrix_set(ttad_matrix* M, const double* A, long long lda, int src_device, void* stream) {
     if (!M || !A || lda < std::max(1, M->n)) return set_error(TTAD_ERR_INVALID, "matrix_set: bad args");
+    M->anorm_valid = false;
     hipStream_t s = pick(M->ctx, stream);
     if (src_device) {
         launch_pack(M->storage, A, lda, M->desc.as<DevMat>(), M->nt, s);
@@ -375,6 +399,7 @@ int ttad_matrix_get(ttad_matrix* M, double* A, long long lda, int dst_device, vo
 int ttad_matrix_copy(ttad_matrix* dst, const ttad_matrix* src, void* stream) {
     if (!dst || !src || dst->nt != src->nt || dst->bsz != src->bsz)
         return set_error(TTAD_ERR_INVALID, "matrix_copy: partitions differ");
+    dst->anorm_valid = false;
     launch_copy_tiles(dst->storage, src->storage, src->desc.as<DevMat>(), dst->desc.as<DevMat>(), dst->nt,
                       pick(dst->ctx, stream));
     TTAD_HIP(hipGetLastError());
@@ -403,7 +428,7 @@ int ttad_matrix_tile(const ttad_matrix* M, int i, int j, void** ptr, int* ld) {
 int ttad_potrf_tiled(ttad_context* ctx, ttad_matrix* M, int schedule, int prio_kind, int prec, void* stream,
                      int* info) {
     if (!ctx || !M || prio_kind < 0 || prio_kind > 3) return set_error(TTAD_ERR_INVALID, "potrf_tiled: bad args");
-    if ((prec == TTAD_F64) != (M->storage == TTAD_F64) || prec < 0 || prec > 2)
+    if ((prec == TTAD_F64) != (M->storage == TTAD_F64) || prec < 0 || prec > 3)
         return set_error(TTAD_ERR_INVALID, "potrf_tiled: precision %d does not match the tile storage", prec);
     if (info) *info = 0;
     hipStream_t s = pick(ctx, stream);
@@ -474,7 +499,7 @@ int ttad_graph_create(ttad_context* ctx, int n_tasks, const ttad_task_desc* task
         if (g->out_ptr[t + 1] < g->out_ptr[t]) return set_error(TTAD_ERR_INVALID, "graph_create: out_ptr not sorted");
         if (indegree[t] < 1) return set_error(TTAD_ERR_GRAPH, "graph_create: task %d has indegree < 1", t);
         const ttad_task_desc& d = tasks[t];
-        if (d.kind < TTAD_POTRF || d.kind > TTAD_GEMM || d.prec < 0 || d.prec > 2 || d.mat < 0 || d.mat > 65535)
+        if (d.kind < TTAD_POTRF || d.kind > TTAD_GEMM || d.prec < 0 || d.prec > 3 || d.mat < 0 || d.mat > 65535)
             return set_error(TTAD_ERR_INVALID, "graph_create: bad descriptor for task %d", t);
         DevTask& x = g->tasks[t];
         x.kind = d.kind;

@@ -77,10 +77,14 @@ __device__ __forceinline__ void exec_item(const DevTask& t, int it, const DevMat
                 item_gemm<float, P_F32>((const float*)tile(i, k), m, (const float*)tile(j, k), n,
                                         (float*)tile(i, j), m, m, n, K, rb * GEMM_BM, cb * GEMM_BM, syrk, -1.0, 1.0,
                                         lds);
-            else
+            else if (t.prec == P_BF16)
                 item_gemm<float, P_BF16>((const float*)tile(i, k), m, (const float*)tile(j, k), n,
                                          (float*)tile(i, j), m, m, n, K, rb * GEMM_BM, cb * GEMM_BM, syrk, -1.0,
                                          1.0, lds);
+            else
+                item_gemm<float, P_BF16X3>((const float*)tile(i, k), m, (const float*)tile(j, k), n,
+                                           (float*)tile(i, j), m, m, n, K, rb * GEMM_BM, cb * GEMM_BM, syrk, -1.0,
+                                           1.0, lds);
             break;
         }
         default:

@@ -285,6 +285,7 @@ hipError_t kernels_init() {
     TTAD_SETLDS((k_raw_gemm<double, P_F64>));
     TTAD_SETLDS((k_raw_gemm<float, P_F32>));
     TTAD_SETLDS((k_raw_gemm<float, P_BF16>));
+    TTAD_SETLDS((k_raw_gemm<float, P_BF16X3>));
     TTAD_SETLDS((k_raw_potrf<double>));
     TTAD_SETLDS((k_raw_potrf<float>));
     TTAD_SETLDS((k_raw_trsm<double, double>));
@@ -327,8 +328,11 @@ void launch_raw_gemm(int prec, const void* A, long lda, const void* B, long ldb,
     else if (prec == P_F32)
         hipLaunchKernelGGL((k_raw_gemm<float, P_F32>), dim3(items), dim3(NTHREADS), LDS_ITEM_MAX, s, (const float*)A,
                            lda, (const float*)B, ldb, (float*)C, ldc, m, n, K, syrk, alpha, beta);
-    else
+    else if (prec == P_BF16)
         hipLaunchKernelGGL((k_raw_gemm<float, P_BF16>), dim3(items), dim3(NTHREADS), LDS_ITEM_MAX, s,
+                           (const float*)A, lda, (const float*)B, ldb, (float*)C, ldc, m, n, K, syrk, alpha, beta);
+    else
+        hipLaunchKernelGGL((k_raw_gemm<float, P_BF16X3>), dim3(items), dim3(NTHREADS), LDS_ITEM_MAX, s,
                            (const float*)A, lda, (const float*)B, ldb, (float*)C, ldc, m, n, K, syrk, alpha, beta);
 }
 

@@ -68,6 +68,10 @@ struct ttad_context {
     long long next_matrix_id = 1;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;  // bracket the last ttad_potrf_tiled on its stream
     bool ev_valid = false;
+    // solve-phase workspace (grown on demand)
+    ttad::DevBuf sw_Y, sw_D, sw_R, sw_flags, sw_ticket, sw_err, sw_norms;
+    // working copies of FP64 matrices for ttad_posv_mixed, keyed by (matrix id, storage)
+    std::map<std::pair<long long, int>, ttad_matrix*> work_mats;
 };
 
 struct ttad_matrix {
@@ -80,6 +84,11 @@ struct ttad_matrix {
     size_t elem = 8;
     long long bytes = 0;
     ttad::DevMat host_desc{};
+    // 64-row block table of the solve phase (built on first use)
+    int sweep_P = 0;
+    ttad::DevBuf tab_blk_tile, tab_blk_off;
+    double anorm = 0.0;  // ||A||_inf, valid until the next ttad_matrix_set/copy into this matrix
+    bool anorm_valid = false;
 };
 
 struct ttad_graph {

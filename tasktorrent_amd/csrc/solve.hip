@@ -1,0 +1,310 @@
+// Solve phase: triangular solves with a tiled factor, FP64 residual, iterative
+// refinement and the mixed-precision solver with FP64 fallback.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "runtime.hpp"
+#include "solve_ops.hpp"
+
+namespace ttad {
+
+// Per-matrix 64-row block table (built on first use).
+static int ensure_sweep_tab(ttad_matrix* M) {
+    if (M->sweep_P > 0) return TTAD_OK;
+    std::vector<int32_t> bt, bo;
+    for (int i = 0; i < M->nt; i++) {
+        if (i < M->nt - 1 && M->bsz[i] % SB != 0)
+            return set_error(TTAD_ERR_INVALID, "solve: tile %d has size %d, not a multiple of %d", i, M->bsz[i], SB);
+        for (int o = 0; o < M->bsz[i]; o += SB) {
+            bt.push_back(i);
+            bo.push_back(o);
+        }
+    }
+    TTAD_HIP(M->tab_blk_tile.alloc(std::max<size_t>(1, bt.size()) * 4));
+    TTAD_HIP(M->tab_blk_off.alloc(std::max<size_t>(1, bo.size()) * 4));
+    TTAD_HIP(hipMemcpy(M->tab_blk_tile.p, bt.data(), bt.size() * 4, hipMemcpyHostToDevice));
+    TTAD_HIP(hipMemcpy(M->tab_blk_off.p, bo.data(), bo.size() * 4, hipMemcpyHostToDevice));
+    M->sweep_P = (int)bt.size();
+    return TTAD_OK;
+}
+
+static SweepTab tab_of(const ttad_matrix* M) {
+    return SweepTab{M->sweep_P, M->tab_blk_tile.as<int32_t>(), M->tab_blk_off.as<int32_t>()};
+}
+
+static int ensure(DevBuf& b, size_t bytes) {
+    if (b.bytes < bytes) TTAD_HIP(b.alloc(bytes));
+    return TTAD_OK;
+}
+
+__global__ void k_zero_i32(int32_t* p, int n) {
+    for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < n; x += gridDim.x * blockDim.x) p[x] = 0;
+}
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+    for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < n; x += gridDim.x * blockDim.x) p[x] = 0;
+}
+__global__ void k_zero_cols(double* X, long ld, int n, int nrhs) {
+    const long tot = (long)n * nrhs;
+    for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < tot; x += (long)gridDim.x * blockDim.x)
+        X[(x % n) + (x / n) * ld] = 0.0;
+}
+__global__ void k_copy_cols(const double* S, long lds_, double* D, long ldd, int n, int nrhs) {
+    const long tot = (long)n * nrhs;
+    for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < tot; x += (long)gridDim.x * blockDim.x)
+        D[(x % n) + (x / n) * ldd] = S[(x % n) + (x / n) * lds_];
+}
+
+static int64_t spin_ticks(ttad_context* ctx) {
+    int rate_khz = 100000;
+    (void)hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device);
+    return (int64_t)rate_khz * 1000 * 5;
+}
+
+template <typename TL, typename TC>
+static void launch_sweep(ttad_matrix* L, int forward, const double* R, long ldr, TC* Y, long ldy, TC* D, long ldd,
+                         double* X, long ldx, int nrhs, int32_t* flags, int32_t* ticket, int32_t* err, int64_t spin,
+                         hipStream_t s) {
+    const SweepTab t = tab_of(L);
+    const int P = t.P;
+    hipLaunchKernelGGL(k_zero_i32, dim3((P + 255) / 256 + 1), dim3(256), 0, s, flags, P);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, ticket, 1);
+    const DevMat* Md = L->desc.as<DevMat>();
+    if (nrhs <= 16)
+        hipLaunchKernelGGL((k_sweep<TL, TC, 16>), dim3(P), dim3(NTHREADS), 0, s, Md, t, forward, R, ldr, Y, ldy, D,
+                           ldd, X, ldx, nrhs, flags, ticket, err, spin);
+    else
+        hipLaunchKernelGGL((k_sweep<TL, TC, 32>), dim3(P), dim3(NTHREADS), 0, s, Md, t, forward, R, ldr, Y, ldy, D,
+                           ldd, X, ldx, nrhs, flags, ticket, err, spin);
+}
+
+// X += (L L^T)^{-1} R, in chunks of at most 32 right-hand sides; work precision of L.
+static int solve_add(ttad_context* ctx, ttad_matrix* L, int nrhs, const double* R, long ldr, double* X, long ldx,
+                     hipStream_t s) {
+    int rc = ensure_sweep_tab(L);
+    if (rc) return rc;
+    const int n = L->n, P = L->sweep_P;
+    const size_t es = L->storage == P_F64 ? 8 : 4;
+    const int chunk = std::min(nrhs, 32);
+    if ((rc = ensure(ctx->sw_Y, (size_t)n * chunk * es))) return rc;
+    if ((rc = ensure(ctx->sw_D, (size_t)n * chunk * es))) return rc;
+    if ((rc = ensure(ctx->sw_flags, (size_t)(P + 1) * 4))) return rc;
+    if ((rc = ensure(ctx->sw_ticket, 16))) return rc;
+    if ((rc = ensure(ctx->sw_err, 16))) return rc;
+    const int64_t spin = spin_ticks(ctx);
+    int32_t* flags = ctx->sw_flags.as<int32_t>();
+    int32_t* ticket = ctx->sw_ticket.as<int32_t>();
+    int32_t* err = ctx->sw_err.as<int32_t>();
+    for (int c0 = 0; c0 < nrhs; c0 += chunk) {
+        const int nc = std::min(chunk, nrhs - c0);
+        const double* Rc = R + (long)c0 * ldr;
+        double* Xc = X + (long)c0 * ldx;
+        if (L->storage == P_F64) {
+            double* Y = ctx->sw_Y.as<double>();
+            double* D = ctx->sw_D.as<double>();
+            launch_sweep<double, double>(L, 1, Rc, ldr, Y, n, D, n, Xc, ldx, nc, flags, ticket, err, spin, s);
+            launch_sweep<double, double>(L, 0, Rc, ldr, Y, n, D, n, Xc, ldx, nc, flags, ticket, err, spin, s);
+        } else {
+            float* Y = ctx->sw_Y.as<float>();
+            float* D = ctx->sw_D.as<float>();
+            launch_sweep<float, float>(L, 1, Rc, ldr, Y, n, D, n, Xc, ldx, nc, flags, ticket, err, spin, s);
+            launch_sweep<float, float>(L, 0, Rc, ldr, Y, n, D, n, Xc, ldx, nc, flags, ticket, err, spin, s);
+        }
+    }
+    TTAD_HIP(hipGetLastError());
+    return TTAD_OK;
+}
+
+// r = b - A x and the normwise backward error per column.
+static int residual(ttad_context* ctx, ttad_matrix* A, int nrhs, const double* B, long ldb, const double* X,
+                    long ldx, double* R, long ldr, double* eta_max, hipStream_t s) {
+    int rc = ensure_sweep_tab(A);
+    if (rc) return rc;
+    if (A->storage != P_F64) return set_error(TTAD_ERR_INVALID, "residual: A must have FP64 tiles");
+    const SweepTab t = tab_of(A);
+    const DevMat* Md = A->desc.as<DevMat>();
+    if ((rc = ensure(ctx->sw_norms, (size_t)(3 * 32 + 8) * 8))) return rc;
+    unsigned long long* norms = ctx->sw_norms.as<unsigned long long>();
+    if (!A->anorm_valid) {
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(128), 0, s, norms, 3 * 32 + 8);
+        hipLaunchKernelGGL((k_residual<16>), dim3(t.P), dim3(NTHREADS), 0, s, Md, t, B, ldb, X, ldx, R, ldr, 1,
+                           norms, 1);
+        unsigned long long h = 0;
+        TTAD_HIP(hipMemcpyAsync(&h, norms, 8, hipMemcpyDeviceToHost, s));
+        TTAD_HIP(hipStreamSynchronize(s));
+        double v;
+        memcpy(&v, &h, 8);
+        A->anorm = v;
+        A->anorm_valid = true;
+    }
+    double worst = 0.0;
+    for (int c0 = 0; c0 < nrhs; c0 += 32) {
+        const int nc = std::min(32, nrhs - c0);
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(128), 0, s, norms, 3 * 32 + 8);
+        if (nc <= 16)
+            hipLaunchKernelGGL((k_residual<16>), dim3(t.P), dim3(NTHREADS), 0, s, Md, t, B + (long)c0 * ldb, ldb,
+                               X + (long)c0 * ldx, ldx, R + (long)c0 * ldr, ldr, nc, norms, 0);
+        else
+            hipLaunchKernelGGL((k_residual<32>), dim3(t.P), dim3(NTHREADS), 0, s, Md, t, B + (long)c0 * ldb, ldb,
+                               X + (long)c0 * ldx, ldx, R + (long)c0 * ldr, ldr, nc, norms, 0);
+        TTAD_HIP(hipGetLastError());
+        unsigned long long h[3 * 32];
+        TTAD_HIP(hipMemcpyAsync(h, norms, sizeof(unsigned long long) * 3 * nc, hipMemcpyDeviceToHost, s));
+        TTAD_HIP(hipStreamSynchronize(s));
+        for (int j = 0; j < nc; j++) {
+            double rn, xn, bn;
+            memcpy(&rn, &h[3 * j], 8);
+            memcpy(&xn, &h[3 * j + 1], 8);
+            memcpy(&bn, &h[3 * j + 2], 8);
+            const double den = A->anorm * xn + bn;
+            const double eta = den > 0 ? rn / den : (rn > 0 ? INFINITY : 0.0);
+            worst = std::max(worst, std::isnan(eta) ? INFINITY : eta);
+        }
+    }
+    if (eta_max) *eta_max = worst;
+    return TTAD_OK;
+}
+
+static int check_err_flag(ttad_context* ctx, hipStream_t s) {
+    int32_t e = 0;
+    TTAD_HIP(hipMemcpyAsync(&e, ctx->sw_err.p, 4, hipMemcpyDeviceToHost, s));
+    TTAD_HIP(hipStreamSynchronize(s));
+    if (e) return set_error(TTAD_ERR_TIMEOUT, "solve: a sweep block waited past the device watchdog");
+    return TTAD_OK;
+}
+
+// Refinement loop: X := X + L^{-T}L^{-1}(B - A X) until eta <= tol; stops early when
+// eta no longer halves (stall). Returns TTAD_ERR_IR_STALLED if tol is not reached.
+static int refine(ttad_context* ctx, ttad_matrix* A, ttad_matrix* L, int nrhs, const double* B, long ldb, double* X,
+                  long ldx, double tol, int max_iter, int* iters, double* eta_out, hipStream_t s) {
+    const int n = A->n;
+    int rc = ensure(ctx->sw_R, (size_t)n * nrhs * 8);
+    if (rc) return rc;
+    double* R = ctx->sw_R.as<double>();
+    double eta = INFINITY, prev = INFINITY;
+    int it = 0, stall = 0;
+    if ((rc = residual(ctx, A, nrhs, B, ldb, X, ldx, R, n, &eta, s))) return rc;
+    while (eta > tol && it < max_iter) {
+        if ((rc = solve_add(ctx, L, nrhs, R, n, X, ldx, s))) return rc;
+        it++;
+        prev = eta;
+        if ((rc = residual(ctx, A, nrhs, B, ldb, X, ldx, R, n, &eta, s))) return rc;
+        if (!(eta <= 0.5 * prev)) {
+            if (++stall >= 2) break;
+        } else {
+            stall = 0;
+        }
+    }
+    if ((rc = check_err_flag(ctx, s))) return rc;
+    if (iters) *iters = it;
+    if (eta_out) *eta_out = eta;
+    return eta <= tol ? TTAD_OK : set_error(TTAD_ERR_IR_STALLED, "refinement reached eta=%.3e > tol=%.1e after %d steps",
+                                            eta, tol, it);
+}
+
+// Working copy of A (storage st) for the factorization, cached per (A, storage).
+static ttad_matrix* working(ttad_context* ctx, ttad_matrix* A, int st, int& rc) {
+    auto key = std::make_pair(A->id, st);
+    auto it = ctx->work_mats.find(key);
+    if (it != ctx->work_mats.end()) return it->second;
+    ttad_matrix* W = nullptr;
+    rc = ttad_matrix_create(ctx, A->nt, A->bsz.data(), st, &W);
+    if (rc != TTAD_OK) return nullptr;
+    ctx->work_mats[key] = W;
+    return W;
+}
+
+}  // namespace ttad
+
+using namespace ttad;
+
+extern "C" {
+
+int ttad_solve(ttad_context* ctx, ttad_matrix* L, int nrhs, const double* B, long long ldb, double* X, long long ldx,
+               void* stream) {
+    if (!ctx || !L || nrhs <= 0 || !B || !X || ldb < L->n || ldx < L->n)
+        return set_error(TTAD_ERR_INVALID, "solve: bad args");
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    hipLaunchKernelGGL(k_zero_cols, dim3(256), dim3(256), 0, s, X, ldx, L->n, nrhs);
+    int rc = solve_add(ctx, L, nrhs, B, ldb, X, ldx, s);
+    if (rc) return rc;
+    return check_err_flag(ctx, s);
+}
+
+int ttad_residual(ttad_context* ctx, ttad_matrix* A, int nrhs, const double* B, long long ldb, const double* X,
+                  long long ldx, double* R, long long ldr, double* eta, void* stream) {
+    if (!ctx || !A || nrhs <= 0 || !B || !X || !R || ldb < A->n || ldx < A->n || ldr < A->n)
+        return set_error(TTAD_ERR_INVALID, "residual: bad args");
+    return residual(ctx, A, nrhs, B, ldb, X, ldx, R, ldr, eta, stream ? (hipStream_t)stream : ctx->stream);
+}
+
+int ttad_solve_ir(ttad_context* ctx, ttad_matrix* A, ttad_matrix* L, int nrhs, const double* B, long long ldb,
+                  double* X, long long ldx, double tol, int max_iter, ttad_solve_stats* st, void* stream) {
+    if (!ctx || !A || !L || A->nt != L->nt || A->bsz != L->bsz || nrhs <= 0 || !B || !X || ldb < A->n || ldx < A->n)
+        return set_error(TTAD_ERR_INVALID, "solve_ir: bad args");
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    hipLaunchKernelGGL(k_zero_cols, dim3(256), dim3(256), 0, s, X, ldx, A->n, nrhs);
+    int iters = 0;
+    double eta = 0;
+    const int rc = refine(ctx, A, L, nrhs, B, ldb, X, ldx, tol, max_iter, &iters, &eta, s);
+    if (st) {
+        memset(st, 0, sizeof(*st));
+        st->iterations = iters;
+        st->backward_error = eta;
+        st->prec_used = L->storage;
+    }
+    return rc;
+}
+
+int ttad_posv_mixed(ttad_context* ctx, ttad_matrix* A, int prec, int nrhs, const double* B, long long ldb, double* X,
+                    long long ldx, double tol, int max_iter, ttad_solve_stats* st, void* stream) {
+    if (!ctx || !A || A->storage != P_F64 || prec < 0 || prec > 3 || nrhs <= 0 || !B || !X || ldb < A->n ||
+        ldx < A->n)
+        return set_error(TTAD_ERR_INVALID, "posv_mixed: bad args");
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    auto t0 = std::chrono::steady_clock::now();
+    auto secs = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+    int rc = TTAD_OK;
+    ttad_solve_stats loc{};
+    loc.prec_used = prec;
+    ttad_matrix* W = working(ctx, A, prec == P_F64 ? P_F64 : P_F32, rc);
+    if (!W) return rc;
+    if ((rc = ttad_matrix_copy(W, A, s))) return rc;
+    int info = 0;
+    rc = ttad_potrf_tiled(ctx, W, TTAD_SCHED_PERSISTENT, 2, prec, s, &info);
+    loc.factor_s = secs();
+    loc.info = info;
+    int iters = 0;
+    double eta = INFINITY;
+    if (rc == TTAD_OK) {
+        hipLaunchKernelGGL(k_zero_cols, dim3(256), dim3(256), 0, s, X, ldx, A->n, nrhs);
+        rc = refine(ctx, A, W, nrhs, B, ldb, X, ldx, tol, max_iter, &iters, &eta, s);
+    }
+    loc.iterations = iters;
+    loc.backward_error = eta;
+    if ((rc == TTAD_ERR_NOT_SPD || rc == TTAD_ERR_IR_STALLED) && prec != P_F64) {
+        // automatic fallback: FP64 factorization, then refinement with it
+        loc.fell_back = 1;
+        ttad_matrix* W64 = working(ctx, A, P_F64, rc);
+        if (!W64) return rc;
+        if ((rc = ttad_matrix_copy(W64, A, s))) return rc;
+        rc = ttad_potrf_tiled(ctx, W64, TTAD_SCHED_PERSISTENT, 2, P_F64, s, &info);
+        loc.info = info;
+        if (rc == TTAD_OK) {
+            hipLaunchKernelGGL(k_zero_cols, dim3(256), dim3(256), 0, s, X, ldx, A->n, nrhs);
+            int it2 = 0;
+            rc = refine(ctx, A, W64, nrhs, B, ldb, X, ldx, tol, max_iter, &it2, &eta, s);
+            loc.iterations += it2;
+            loc.backward_error = eta;
+        }
+        loc.prec_used = P_F64;
+    }
+    TTAD_HIP(hipStreamSynchronize(s));
+    loc.total_s = secs();
+    loc.solve_s = loc.total_s - loc.factor_s;
+    if (st) *st = loc;
+    return rc;
+}
+
+}  // extern "C"

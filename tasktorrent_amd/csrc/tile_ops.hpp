@@ -21,12 +21,14 @@ typedef short s8_t __attribute__((ext_vector_type(8)));
 // MFMA 16x16 primitives. Operand layouts (lane l of the wave):
 //   f64/f32 16x16x4 : A[m = l%16][k = l/16], B[k = l/16][n = l%16]
 //   bf16   16x16x32 : A[m = l%16][k = 8(l/16)+e], B[k = 8(l/16)+e][n = l%16], e = 0..7
-//   result          : D[m = 4(l/16)+r][n = l%16], r = 0..3
+//   result f32/bf16 : D[m = 4(l/16)+r][n = l%16], r = 0..3
+//   result f64      : D[m = (l/16)+4r][n = l%16], r = 0..3  (measured on gfx950)
 // ---------------------------------------------------------------------------
 template <typename TC> struct Mfma;
 template <> struct Mfma<double> {
     typedef d4_t acc_t;
     static constexpr int KSTEP = 4;
+    static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
     static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
@@ -34,6 +36,7 @@ template <> struct Mfma<double> {
 template <> struct Mfma<float> {
     typedef f4_t acc_t;
     static constexpr int KSTEP = 4;
+    static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
     static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
@@ -77,7 +80,7 @@ struct GemmNT {
     // Element (row, col) of the output block held by this lane in acc[a][b][r].
     static __device__ __forceinline__ int row_of(int a, int r) {
         const int lane = threadIdx.x & 63, wr = (threadIdx.x >> 6) >> 1;
-        return wr * (WM * 16) + a * 16 + 4 * (lane >> 4) + r;
+        return wr * (WM * 16) + a * 16 + Mfma<TC>::row(lane, r);
     }
     static __device__ __forceinline__ int col_of(int b) {
         const int lane = threadIdx.x & 63, wc = (threadIdx.x >> 6) & 1;
@@ -128,7 +131,7 @@ struct GemmNT {
                 load_chunk<TA, EA, KS_A>(ra, Ab, lda, a_ok, k0 + ra_k, K);
                 load_chunk<TB, EB, KS_B>(rb, Bb, ldb, b_ok, k0 + rb_k, K);
             }
-#pragma unroll
+#pragma unroll 2
             for (int ks = 0; ks < BK / 4; ks++) {
                 const int kr = ks * 4 + (lane >> 4);
                 TC av[WM], bv[WN];
@@ -151,13 +154,18 @@ struct GemmNT {
 // LDS holds bf16 in [row][k] order so that one lane reads its 8 consecutive k
 // values with a single 16-byte load.
 // ---------------------------------------------------------------------------
-template <typename TA, int BM, int BN>
+// SPLIT = true: each FP32 operand x is split into hi = bf16(x) and lo = bf16(x - hi)
+// and the product accumulates hi*hi + hi*lo + lo*hi (three MFMAs), which keeps about
+// 16 significant bits of every operand (Henry, Tang, Heinecke, "Leveraging the
+// bfloat16 artificial intelligence datatype for higher-precision computations", 2019).
+template <typename TA, int BM, int BN, bool SPLIT = false>
 struct GemmNTbf16 {
     static constexpr int BK = 32;
     static constexpr int PK = BK + 8;  // bf16 elements per LDS row (16-byte aligned, skewed)
     static constexpr int WM = BM / 32;
     static constexpr int WN = BN / 32;
-    static constexpr int LDS_BYTES = (BM + BN) * PK * 2;
+    static constexpr int NP = SPLIT ? 2 : 1;  // hi (and lo) planes
+    static constexpr int LDS_BYTES = NP * (BM + BN) * PK * 2;
     typedef f4_t acc_t;
     // each thread converts 8 consecutive k of one row per step
     static constexpr int GA = BM * (BK / 8) / NTHREADS;
@@ -180,21 +188,27 @@ struct GemmNTbf16 {
     }
 
     template <int G, int GS>
-    static __device__ __forceinline__ void load8(s8_t (&reg)[G], const TA* __restrict__ base, long ld, bool row_ok,
-                                                 int kfirst, int K) {
+    static __device__ __forceinline__ void load8(s8_t (&hi)[G], s8_t (&lo)[G], const TA* __restrict__ base, long ld,
+                                                 bool row_ok, int kfirst, int K) {
 #pragma unroll
         for (int s = 0; s < G; s++) {
             const int k = kfirst + s * GS * 8;
 #pragma unroll
-            for (int q = 0; q < 8; q++)
-                reg[s][q] = (row_ok && k + q < K) ? (short)f2bf_bits((float)base[(long)(k + q) * ld]) : (short)0;
+            for (int q = 0; q < 8; q++) {
+                const float x = (row_ok && k + q < K) ? (float)base[(long)(k + q) * ld] : 0.0f;
+                const unsigned short h = f2bf_bits(x);
+                hi[s][q] = (short)h;
+                if constexpr (SPLIT) lo[s][q] = (short)f2bf_bits(x - __uint_as_float((unsigned)h << 16));
+            }
         }
     }
 
     static __device__ __forceinline__ void run(acc_t (&acc)[WM][WN], const TA* __restrict__ A, long lda, int mv,
                                const TA* __restrict__ B, long ldb, int nv, int K, char* lds) {
-        unsigned short* As = reinterpret_cast<unsigned short*>(lds);
-        unsigned short* Bs = As + BM * PK;
+        unsigned short* As = reinterpret_cast<unsigned short*>(lds);  // hi plane of A
+        unsigned short* Bs = As + BM * PK;                            // hi plane of B
+        unsigned short* Al = Bs + BN * PK;                            // lo planes (SPLIT)
+        unsigned short* Bl = Al + BM * PK;
         const int tid = threadIdx.x;
         const int lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
         // thread -> fixed row (tid % BM) and k-groups of 8 (tid / BM + s * NTHREADS / BM)
@@ -202,28 +216,34 @@ struct GemmNTbf16 {
         const bool a_ok = a_row < mv, b_ok = b_row < nv;
         const TA* Ab = A + a_row;
         const TA* Bb = B + b_row;
-        s8_t ra[GA], rb[GB];
+        s8_t ra[GA], rb[GB], la[GA], lb[GB];
         const int nchunks = (K + BK - 1) / BK;
         if (nchunks > 0) {
-            load8<GA, NTHREADS / BM>(ra, Ab, lda, a_ok, a_g * 8, K);
-            load8<GB, NTHREADS / BN>(rb, Bb, ldb, b_ok, b_g * 8, K);
+            load8<GA, NTHREADS / BM>(ra, la, Ab, lda, a_ok, a_g * 8, K);
+            load8<GB, NTHREADS / BN>(rb, lb, Bb, ldb, b_ok, b_g * 8, K);
         }
         for (int c = 0; c < nchunks; c++) {
             __syncthreads();
 #pragma unroll
-            for (int s = 0; s < GA; s++)
-                *reinterpret_cast<s8_t*>(As + a_row * PK + (a_g + s * (NTHREADS / BM)) * 8) = ra[s];
+            for (int s = 0; s < GA; s++) {
+                const int off = a_row * PK + (a_g + s * (NTHREADS / BM)) * 8;
+                *reinterpret_cast<s8_t*>(As + off) = ra[s];
+                if constexpr (SPLIT) *reinterpret_cast<s8_t*>(Al + off) = la[s];
+            }
 #pragma unroll
-            for (int s = 0; s < GB; s++)
-                *reinterpret_cast<s8_t*>(Bs + b_row * PK + (b_g + s * (NTHREADS / BN)) * 8) = rb[s];
+            for (int s = 0; s < GB; s++) {
+                const int off = b_row * PK + (b_g + s * (NTHREADS / BN)) * 8;
+                *reinterpret_cast<s8_t*>(Bs + off) = rb[s];
+                if constexpr (SPLIT) *reinterpret_cast<s8_t*>(Bl + off) = lb[s];
+            }
             __syncthreads();
             if (c + 1 < nchunks) {
                 const int k0 = (c + 1) * BK;
-                load8<GA, NTHREADS / BM>(ra, Ab, lda, a_ok, k0 + a_g * 8, K);
-                load8<GB, NTHREADS / BN>(rb, Bb, ldb, b_ok, k0 + b_g * 8, K);
+                load8<GA, NTHREADS / BM>(ra, la, Ab, lda, a_ok, k0 + a_g * 8, K);
+                load8<GB, NTHREADS / BN>(rb, lb, Bb, ldb, b_ok, k0 + b_g * 8, K);
             }
-            s8_t av[WM], bv[WN];
             const int kg = (lane >> 4) * 8;
+            s8_t av[WM], bv[WN];
 #pragma unroll
             for (int a = 0; a < WM; a++)
                 av[a] = *reinterpret_cast<const s8_t*>(As + (wr * (WM * 16) + a * 16 + (lane & 15)) * PK + kg);
@@ -235,6 +255,22 @@ struct GemmNTbf16 {
 #pragma unroll
                 for (int b = 0; b < WN; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
+            if constexpr (SPLIT) {
+                s8_t avl[WM], bvl[WN];
+#pragma unroll
+                for (int a = 0; a < WM; a++)
+                    avl[a] = *reinterpret_cast<const s8_t*>(Al + (wr * (WM * 16) + a * 16 + (lane & 15)) * PK + kg);
+#pragma unroll
+                for (int b = 0; b < WN; b++)
+                    bvl[b] = *reinterpret_cast<const s8_t*>(Bl + (wc * (WN * 16) + b * 16 + (lane & 15)) * PK + kg);
+#pragma unroll
+                for (int a = 0; a < WM; a++)
+#pragma unroll
+                    for (int b = 0; b < WN; b++) {
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bvl[b], acc[a][b], 0, 0, 0);
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(avl[a], bv[b], acc[a][b], 0, 0, 0);
+                    }
+            }
         }
         __syncthreads();
     }
@@ -252,8 +288,8 @@ __device__ __forceinline__ void item_gemm(const TS* __restrict__ A, long lda, co
                           char* lds) {
     const int mv = min(GEMM_BM, m - r0), nv = min(GEMM_BM, n - c0);
     if (mv <= 0 || nv <= 0) return;
-    if constexpr (PREC == P_BF16) {
-        using G = GemmNTbf16<TS, GEMM_BM, GEMM_BM>;
+    if constexpr (PREC == P_BF16 || PREC == P_BF16X3) {
+        using G = GemmNTbf16<TS, GEMM_BM, GEMM_BM, PREC == P_BF16X3>;
         typename G::acc_t acc[G::WM][G::WN];
         G::zero(acc);
         G::run(acc, A + r0, lda, mv, B + c0, ldb, nv, K, lds);
@@ -504,7 +540,7 @@ __device__ __forceinline__ void item_trsm(const TS* __restrict__ L, long ldl, co
 // LDS bytes each item kind needs (the executor reserves the maximum).
 constexpr int LDS_GEMM_F64 = GemmNT<double, double, double, GEMM_BM, GEMM_BM, 16>::LDS_BYTES;
 constexpr int LDS_GEMM_F32 = GemmNT<float, float, float, GEMM_BM, GEMM_BM, 16>::LDS_BYTES;
-constexpr int LDS_GEMM_BF16 = GemmNTbf16<float, GEMM_BM, GEMM_BM>::LDS_BYTES;
+constexpr int LDS_GEMM_BF16 = GemmNTbf16<float, GEMM_BM, GEMM_BM, true>::LDS_BYTES;
 constexpr int LDS_POTRF = GemmNT<double, double, double, 64, 64, 32>::LDS_BYTES + DB * DP * 8;
 constexpr int LDS_TRSM = GemmNT<double, double, double, 64, 64, 32>::LDS_BYTES;
 constexpr int cmax(int a, int b) { return a > b ? a : b; }

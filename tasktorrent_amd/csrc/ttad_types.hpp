@@ -7,7 +7,7 @@ namespace ttad {
 // Task kinds of the tiled Cholesky PTG (miniapp/dense_cholesky/2d_cholesky.cpp:316-608).
 enum Kind : int32_t { K_POTRF = 0, K_TRSM = 1, K_SYRK = 2, K_GEMM = 3, K_ACCU = 4, K_NOP = 5 };
 // Arithmetic of a task. Tiles are stored in FP64 (P_F64) or FP32 (the mixed variants).
-enum Prec : int32_t { P_F64 = 0, P_F32 = 1, P_BF16 = 2 };
+enum Prec : int32_t { P_F64 = 0, P_F32 = 1, P_BF16 = 2, P_BF16X3 = 3 };
 
 constexpr int NTHREADS = 256;  // one workgroup = 4 wave64
 constexpr int GEMM_BM = 128;   // GEMM/SYRK work item = 128x128 output sub-tile

@@ -1,0 +1,114 @@
+"""Solve-phase parity on the GPU.
+
+- ttad_solve with the FP64 factor against the oracle's two triangular solves
+  (2d_cholesky.cpp:676-681), relative 1e-12 on well-conditioned A;
+- ttad_residual against b - A x computed on the host in FP64 (1e-13 relative) and
+  the normwise backward error definition;
+- mixed precision + iterative refinement reaches eta <= 1e-12 (the bar of the
+  mixed solver) on the ill-conditioned A = D (B B^T/n + delta I) D family, and
+  matches numpy's FP64 solve to the accuracy its condition number allows;
+- the FP64 fallback triggers on a matrix whose low-precision factor breaks down.
+"""
+import numpy as np
+import pytest
+
+import tasktorrent_amd as tt
+from gpu_util import from_dev, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev_cols(X):
+    """n x k column-major host array -> device tensor (k, n) (same layout)."""
+    return to_dev(np.asfortranarray(X))
+
+
+def _mat(ctx, A, b, storage="f64"):
+    n = A.shape[0]
+    sizes = [b] * (n // b) + ([n % b] if n % b else [])
+    M = tt.TiledMatrix(ctx, sizes, storage)
+    M.set(A)
+    return M
+
+
+def backward_error(A, X, B):
+    R = B - A @ X
+    an = np.abs(A).sum(axis=1).max()
+    return float((np.abs(R).max(axis=0) / (an * np.abs(X).max(axis=0) + np.abs(B).max(axis=0))).max())
+
+
+@pytest.mark.parametrize("n,b,nrhs", [(1024, 256, 1), (1536, 512, 16), (640, 128, 20), (704, 192, 3)])
+def test_solve_fp64_factor(ctx, oracle, n, b, nrhs):
+    A = oracle.spd_bbt(n, n + nrhs)
+    M = _mat(ctx, A, b)
+    assert ctx.potrf_tiled(M, tt.TTAD_SCHED_PERSISTENT, 2, "f64") == 0
+    B = np.asfortranarray(np.random.default_rng(nrhs).standard_normal((n, nrhs)))
+    dB, dX = _dev_cols(B), _dev_cols(np.zeros((n, nrhs)))
+    ctx.solve(M, nrhs, dB, dX)
+    X = from_dev(dX)
+    Xref = np.linalg.solve(A, B)
+    assert np.linalg.norm(X - Xref) / np.linalg.norm(Xref) < 1e-12
+    assert backward_error(A, X, B) < 1e-15
+
+
+@pytest.mark.parametrize("nrhs", [1, 16, 40])
+def test_residual(ctx, oracle, nrhs):
+    n, b = 1024, 256
+    A = oracle.spd_scaled(n, 3, 4)
+    M = _mat(ctx, A, b)
+    rng = np.random.default_rng(nrhs)
+    B = np.asfortranarray(rng.standard_normal((n, nrhs)))
+    X = np.asfortranarray(rng.standard_normal((n, nrhs)))
+    dB, dX, dR = _dev_cols(B), _dev_cols(X), _dev_cols(np.zeros((n, nrhs)))
+    eta = ctx.residual(M, nrhs, dB, dX, dR)
+    R = from_dev(dR)
+    Rref = B - A @ X
+    assert np.linalg.norm(R - Rref) / np.linalg.norm(Rref) < 1e-13
+    assert abs(eta - backward_error(A, X, B)) <= 1e-12 * backward_error(A, X, B)
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16x3", "bf16"])
+def test_mixed_ir_reaches_1e12(ctx, oracle, prec):
+    n, b, nrhs = 2048, 256, 16
+    A = oracle.spd_scaled(n, 2022, 2023)  # kappa ~ 1e4 - 1e6
+    M = _mat(ctx, A, b)
+    B = np.asfortranarray(np.random.default_rng(2024).standard_normal((n, nrhs)))
+    dB, dX = _dev_cols(B), _dev_cols(np.zeros((n, nrhs)))
+    st = ctx.posv_mixed(M, prec, nrhs, dB, dX, tol=1e-12, max_iter=30)
+    X = from_dev(dX)
+    eta = backward_error(A, X, B)
+    assert st["rc"] == 0, st
+    assert st["backward_error"] <= 1e-12 and eta <= 2e-12, (st, eta)
+    Xref = np.linalg.solve(A, B)
+    kappa = np.linalg.cond(A)
+    # forward error of a solution with backward error eta is bounded by about kappa * eta
+    fwd = np.linalg.norm(X - Xref) / np.linalg.norm(Xref)
+    assert fwd < 10 * kappa * max(st["backward_error"], 1e-16) + 1e-13, (fwd, kappa, st)
+
+
+def test_fallback_on_breakdown(ctx, oracle):
+    # kappa ~ 1e12 well beyond what an FP32 factor refines: expect the FP64 fallback
+    n, b, nrhs = 1024, 256, 4
+    A = oracle.spd_scaled(n, 5, 6, delta=1e-9, dmax=1e3)
+    M = _mat(ctx, A, b)
+    B = np.asfortranarray(np.random.default_rng(1).standard_normal((n, nrhs)))
+    dB, dX = _dev_cols(B), _dev_cols(np.zeros((n, nrhs)))
+    st = ctx.posv_mixed(M, "bf16", nrhs, dB, dX, tol=1e-12, max_iter=10)
+    assert st["fell_back"] == 1
+    assert st["prec_used"] == tt.TTAD_F64
+    if st["rc"] == 0:
+        assert backward_error(A, from_dev(dX), B) <= 2e-12
+
+
+def test_factor_reused_for_new_rhs(ctx, oracle):
+    n, b = 1024, 256
+    A = oracle.spd_bbt(n, 8)
+    M = _mat(ctx, A, b)
+    W = tt.TiledMatrix(ctx, M.block_sizes, "f32")
+    W.copy_from(M)
+    assert ctx.potrf_tiled(W, tt.TTAD_SCHED_PERSISTENT, 2, "f32") == 0
+    for seed in range(3):
+        B = np.asfortranarray(np.random.default_rng(seed).standard_normal((n, 2)))
+        dB, dX = _dev_cols(B), _dev_cols(np.zeros((n, 2)))
+        st = ctx.solve_ir(M, W, 2, dB, dX, tol=1e-13, max_iter=20)
+        assert st["rc"] == 0 and backward_error(A, from_dev(dX), B) <= 2e-13
