@@ -201,6 +201,51 @@ int ttad_solve_ir(ttad_context* ctx, ttad_matrix* A, ttad_matrix* L, int nrhs, c
 int ttad_posv_mixed(ttad_context* ctx, ttad_matrix* A, int prec, int nrhs, const double* B, long long ldb, double* X,
                     long long ldx, double tol, int max_iter, ttad_solve_stats* stats, void* stream);
 
+/* ---- serving loop: device-resident continuous batching of SPD solves ---------------- */
+/* No reference counterpart (the reference runs one factorization per process). A
+ * server owns a persistent kernel that runs until ttad_server_destroy; requests posted
+ * with ttad_server_submit are admitted while it runs, as their HBM footprint (factor +
+ * vectors, from an arena of hbm_budget bytes), task-pool range and request slot become
+ * free, and their task graphs share the same workgroups. */
+typedef struct ttad_server ttad_server;
+typedef struct {
+    long long hbm_budget;  /* arena bytes for factors and vectors (admission limit) */
+    int max_tasks;         /* task pool size */
+    int max_requests;      /* concurrent request slots */
+    int workers;           /* persistent workgroups; <= 0: all resident slots */
+    int idle_timeout_s;    /* the device loop exits after this long without work (default 600) */
+} ttad_server_config;
+typedef struct {
+    long long id;
+    int status;             /* TTAD_OK, TTAD_ERR_IR_STALLED, TTAD_ERR_NOT_SPD, or an admission error */
+    int n, nrhs;
+    int iterations;         /* refinement steps */
+    int reused;             /* 1 if the request used a cached factor */
+    long long factor_id;    /* id of the factor used (pass as reuse_factor to solve again) */
+    double backward_error;  /* normwise backward error of the returned solution */
+    double latency_s;       /* submit -> solution copied back to the caller's buffer */
+    double queue_s;         /* submit -> admission */
+} ttad_server_result;
+typedef struct {
+    int pending, active, cached_factors, workers;
+    long long arena_used, arena_size, items_run;
+} ttad_server_stats;
+
+int ttad_server_create(ttad_context* ctx, const ttad_server_config* cfg, ttad_server** out);
+/* A: FP64 tiled matrix (tile sizes multiples of 64), resident and unchanged until the
+ * request completes. B: host n x nrhs column-major (copied at admission); X: host
+ * buffer that receives the solution when ttad_server_poll reports the request.
+ * prec: TTAD_F64 / TTAD_F32 / TTAD_BF16 / TTAD_BF16X3 factorization; reuse_factor >= 0
+ * solves with a factor kept by an earlier request (keep_factor != 0) instead of
+ * factoring. Never blocks: requests that do not fit yet wait in a host queue. */
+int ttad_server_submit(ttad_server* S, ttad_matrix* A, int prec, int nrhs, const double* B, double* X, double tol,
+                       int max_iter, long long reuse_factor, int keep_factor, long long* id);
+/* Collects completed requests (up to max_out) and admits waiting ones. */
+int ttad_server_poll(ttad_server* S, ttad_server_result* out, int max_out, int* n_out);
+int ttad_server_query(ttad_server* S, ttad_server_stats* stats);
+/* Stops the device loop after the requests in flight complete, frees everything. */
+int ttad_server_destroy(ttad_server* S);
+
 /* The Cholesky PTG of 2d_cholesky.cpp:281-614 (one rank, non-accumulate mode),
  * declared through the Taskflow drop-in and compiled on the host without a
  * device: tasks (POTRF/TRSM/SYRK/GEMM with tile coordinates, k = step), their

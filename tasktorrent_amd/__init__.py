@@ -19,7 +19,7 @@ from ._capi import (TTAD_BF16, TTAD_BF16X3, TTAD_F32, TTAD_F64, TTAD_GEMM, TTAD_
                     TTAD_SCHED_STREAM, TTAD_SYRK, TTAD_TRSM, TtadError, lib, lib_path)
 
 __all__ = [
-    "Context", "TiledMatrix", "Graph", "TtadError", "lib", "lib_path",
+    "Context", "TiledMatrix", "Graph", "Server", "TtadError", "lib", "lib_path",
     "TTAD_F64", "TTAD_F32", "TTAD_BF16", "TTAD_BF16X3", "TTAD_POTRF", "TTAD_TRSM", "TTAD_SYRK", "TTAD_GEMM",
     "TTAD_SCHED_STREAM", "TTAD_SCHED_PERSISTENT",
 ]
@@ -253,6 +253,75 @@ class Graph:
     def close(self):
         if self._h:
             lib().ttad_graph_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Device-resident serving loop (ttad_server): continuous batching of SPD solves.
+
+    submit() never blocks; poll() returns completed requests as dicts and copies each
+    solution into the numpy array given at submission.
+    """
+
+    def __init__(self, ctx: Context, hbm_budget: int = 4 << 30, max_tasks: int = 1 << 20, max_requests: int = 1024,
+                 workers: int = 0, idle_timeout_s: int = 600):
+        cfg = _capi.ServerConfig(hbm_budget, max_tasks, max_requests, workers, idle_timeout_s)
+        self._h = ctypes.c_void_p()
+        _capi.check(lib().ttad_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(self._h)))
+        self.ctx = ctx
+        self._keep = {}
+
+    def submit(self, A: TiledMatrix, B: np.ndarray, prec="bf16x3", tol=1e-12, max_iter=30, reuse_factor=-1,
+               keep_factor=False):
+        B = np.asfortranarray(B, dtype=np.float64)
+        if B.ndim == 1:
+            B = B.reshape(-1, 1, order="F")
+        X = np.zeros_like(B, order="F")
+        p = PREC[prec] if isinstance(prec, str) else prec
+        rid = ctypes.c_longlong()
+        _capi.check(lib().ttad_server_submit(self._h, A.handle, p, B.shape[1], B.ctypes.data_as(ctypes.c_void_p),
+                                             X.ctypes.data_as(ctypes.c_void_p), tol, max_iter, reuse_factor,
+                                             int(keep_factor), ctypes.byref(rid)))
+        self._keep[rid.value] = (B, X)
+        return rid.value, X
+
+    def poll(self, max_out: int = 256):
+        buf = (_capi.ServerResult * max_out)()
+        n = ctypes.c_int()
+        _capi.check(lib().ttad_server_poll(self._h, buf, max_out, ctypes.byref(n)))
+        out = []
+        for r in buf[:n.value]:
+            d = {f: getattr(r, f) for f, _ in _capi.ServerResult._fields_}
+            kept = self._keep.pop(r.id, None)
+            d["X"] = kept[1] if kept else None
+            out.append(d)
+        return out
+
+    def wait_all(self, ids, timeout_s: float = 120.0):
+        import time
+        want, got, t0 = set(ids), {}, time.time()
+        while want - set(got):
+            for r in self.poll():
+                got[r["id"]] = r
+            if time.time() - t0 > timeout_s:
+                raise TimeoutError(f"server: {len(want - set(got))} requests not done after {timeout_s} s")
+            time.sleep(20e-6)
+        return got
+
+    def stats(self):
+        st = _capi.ServerStats()
+        _capi.check(lib().ttad_server_query(self._h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in _capi.ServerStats._fields_}
+
+    def close(self):
+        if self._h:
+            lib().ttad_server_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

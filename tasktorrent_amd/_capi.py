@@ -50,6 +50,23 @@ class SolveStats(ctypes.Structure):
                 ("info", ctypes.c_int), ("prec_used", ctypes.c_int)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_longlong), ("max_tasks", ctypes.c_int), ("max_requests", ctypes.c_int),
+                ("workers", ctypes.c_int), ("idle_timeout_s", ctypes.c_int)]
+
+
+class ServerResult(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_longlong), ("status", ctypes.c_int), ("n", ctypes.c_int), ("nrhs", ctypes.c_int),
+                ("iterations", ctypes.c_int), ("reused", ctypes.c_int), ("factor_id", ctypes.c_longlong),
+                ("backward_error", ctypes.c_double), ("latency_s", ctypes.c_double), ("queue_s", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("pending", ctypes.c_int), ("active", ctypes.c_int), ("cached_factors", ctypes.c_int),
+                ("workers", ctypes.c_int), ("arena_used", ctypes.c_longlong), ("arena_size", ctypes.c_longlong),
+                ("items_run", ctypes.c_longlong)]
+
+
 _lib = None
 
 V = ctypes.c_void_p
@@ -99,6 +116,11 @@ _SIGS = {
     "ttad_residual": (I, [V, V, I, V, LL, V, LL, V, LL, ctypes.POINTER(D), V]),
     "ttad_solve_ir": (I, [V, V, V, I, V, LL, V, LL, D, I, ctypes.POINTER(SolveStats), V]),
     "ttad_posv_mixed": (I, [V, V, I, I, V, LL, V, LL, D, I, ctypes.POINTER(SolveStats), V]),
+    "ttad_server_create": (I, [V, ctypes.POINTER(ServerConfig), ctypes.POINTER(V)]),
+    "ttad_server_submit": (I, [V, V, I, I, V, V, D, I, LL, I, ctypes.POINTER(LL)]),
+    "ttad_server_poll": (I, [V, ctypes.POINTER(ServerResult), I, PI]),
+    "ttad_server_query": (I, [V, ctypes.POINTER(ServerStats)]),
+    "ttad_server_destroy": (I, [V]),
     "ttad_cholesky_dag": (I, [I, I, PI, PI, ctypes.POINTER(TaskDesc), ctypes.POINTER(ctypes.c_int32),
                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32),
                               ctypes.POINTER(ctypes.c_int32)]),

@@ -108,6 +108,20 @@ int cholesky_persistent(ttad_context* ctx, ttad_matrix* M, int prio_kind, int pr
     return ttad_graph_wait(g, 120.0, nullptr);
 }
 
+// The PTG compiled on the host (no device), for graphs that embed it (serving).
+int cholesky_dag_compile(int nb, int prio_kind, std::vector<ttad_task_desc>& desc, std::vector<int32_t>& indeg,
+                         std::vector<int32_t>& out_ptr, std::vector<int32_t>& out_idx) {
+    Threadpool* tp = nullptr;
+    const int rc = build_cholesky_graph(nullptr, nb, TTAD_F64, TTAD_F64, prio_kind, nullptr, &tp);
+    if (rc != TTAD_OK) return rc;
+    std::unique_ptr<Threadpool> hold(tp);
+    desc = tp->task_descs();
+    indeg = tp->indegrees();
+    out_ptr = tp->out_ptr();
+    out_idx = tp->out_idx();
+    return TTAD_OK;
+}
+
 }  // namespace ttad
 
 extern "C" int ttad_cholesky_dag(int num_blocks, int prio_kind, int* n_tasks, int* n_edges, ttad_task_desc* tasks,

@@ -110,32 +110,27 @@ __device__ __forceinline__ void spin_flag(const int32_t* f, int32_t* err, int64_
     }
 }
 
-// Forward sweep: Y = L^{-1} R  (R FP64, Y in TC). backward: X += L^{-T} Y, D = L^{-T} Y (TC).
-// forward != 0 selects the first, else the second.
+// One 64-row block p of a triangular sweep with the factor held in M.
+// forward != 0: Y[p] = L[p,p]^{-1} (R[p] - sum_{q<p} L[p,q] Y[q])          (R FP64, Y in TC)
+// forward == 0: D[p] = L[p,p]^{-T} (Y[p] - sum_{q>p} L[q,p]^T D[q]), X[p] += D[p] (FP64)
+// Waits (acquire) on flags[q] of every block it reads and publishes flags[p] (release).
+// The caller guarantees that every block this one waits on belongs to a running workgroup.
 template <typename TL, typename TC, int NR>
-__global__ __launch_bounds__(NTHREADS) void k_sweep(const DevMat* Mp, SweepTab tab, int forward, const double* R,
-                                                    long ldr, TC* Y, long ldy, TC* D, long ldd, double* X, long ldx,
-                                                    int nrhs, int32_t* flags, int32_t* ticket, int32_t* err,
-                                                    int64_t spin_limit) {
+__device__ __forceinline__ void item_sweep(const DevMat& M, const SweepTab& tab, int forward, int p, const double* R,
+                                           long ldr, TC* Y, long ldy, TC* D, long ldd, double* X, long ldx, int nrhs,
+                                           int32_t* flags, int32_t* err, int64_t spin_limit, char* lds) {
     constexpr int NB = NR / 16;
     constexpr int LD = SB + SPAD, LDY = NR + SPAD;
-    __shared__ __align__(16) TC Ls[SB * LD];
-    __shared__ __align__(16) TC Ys[SB * LDY];
-    __shared__ int s_t;
-    const DevMat& M = *Mp;
+    TC* Ls = reinterpret_cast<TC*>(lds);
+    TC* Ys = Ls + SB * LD;
     const int P = tab.P;
-    if (threadIdx.x == 0) s_t = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const int t = s_t;
-    if (t >= P) return;
-    const int p = forward ? t : P - 1 - t;
     const int g0 = blk_g0(M, tab, p), w = blk_rows(M, tab, p);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     typename Mfma<TC>::acc_t acc[NB];
 #pragma unroll
     for (int nb = 0; nb < NB; nb++) acc[nb] = typename Mfma<TC>::acc_t{0, 0, 0, 0};
-    const int q_begin = forward ? 0 : P - 1, q_end = forward ? p : p, q_step = forward ? 1 : -1;
-    for (int q = q_begin; q != q_end; q += q_step) {
+    const int q_begin = forward ? 0 : P - 1, q_step = forward ? 1 : -1;
+    for (int q = q_begin; q != p; q += q_step) {
         // the factor block does not depend on the flag: fetch it first
         if (forward) fill_block<TL, TC>(Ls, M, tab, p, q, 0, false);
         else fill_block<TL, TC>(Ls, M, tab, p, q, 1, false);
@@ -192,21 +187,40 @@ __global__ __launch_bounds__(NTHREADS) void k_sweep(const DevMat* Mp, SweepTab t
     }
 }
 
-// Residual r = b - A x (FP64, A symmetric from its lower tiles) for one 64-row block per
-// workgroup, with per-rhs maxima of |r|, |x|, |b| (bit patterns of non-negative doubles
-// are ordered like the values, so an integer atomicMax reduces them). rownorm != 0
-// computes the row sums of |A| instead (x ignored) and reduces their maximum into norms[0].
+template <typename TC, int NR>
+constexpr int sweep_lds_bytes() {
+    return SB * (SB + SPAD) * (int)sizeof(TC) + SB * (NR + SPAD) * (int)sizeof(TC);
+}
+
+// Stand-alone sweep launch: one workgroup per block, blocks claimed through a ticket
+// counter in dependency order.
+template <typename TL, typename TC, int NR>
+__global__ __launch_bounds__(NTHREADS) void k_sweep(const DevMat* Mp, SweepTab tab, int forward, const double* R,
+                                                    long ldr, TC* Y, long ldy, TC* D, long ldd, double* X, long ldx,
+                                                    int nrhs, int32_t* flags, int32_t* ticket, int32_t* err,
+                                                    int64_t spin_limit) {
+    __shared__ __align__(16) char lds[sweep_lds_bytes<TC, NR>()];
+    __shared__ int s_t;
+    if (threadIdx.x == 0) s_t = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int t = s_t;
+    if (t >= tab.P) return;
+    const int p = forward ? t : tab.P - 1 - t;
+    item_sweep<TL, TC, NR>(*Mp, tab, forward, p, R, ldr, Y, ldy, D, ldd, X, ldx, nrhs, flags, err, spin_limit, lds);
+}
+
+// Residual r = b - A x (FP64, A symmetric from its lower tiles) for one 64-row block p,
+// with per-rhs maxima of |r|, |x|, |b| (bit patterns of non-negative doubles are ordered
+// like the values, so an integer atomicMax reduces them). rownorm != 0 computes the row
+// sums of |A| instead (x ignored) and reduces their maximum into norms[0].
 template <int NR>
-__global__ __launch_bounds__(NTHREADS) void k_residual(const DevMat* Mp, SweepTab tab, const double* B, long ldb,
-                                                       const double* X, long ldx, double* R, long ldr, int nrhs,
-                                                       unsigned long long* norms, int rownorm) {
+__device__ __forceinline__ void item_resid(const DevMat& M, const SweepTab& tab, int p, const double* B, long ldb,
+                                           const double* X, long ldx, double* R, long ldr, int nrhs,
+                                           unsigned long long* norms, int rownorm, char* lds) {
     constexpr int NB = NR / 16;
     constexpr int LD = SB + SPAD, LDY = NR + SPAD;
-    __shared__ __align__(16) double Ls[SB * LD];
-    __shared__ __align__(16) double Ys[SB * LDY];
-    const DevMat& M = *Mp;
-    const int p = blockIdx.x;
-    if (p >= tab.P) return;
+    double* Ls = reinterpret_cast<double*>(lds);
+    double* Ys = Ls + SB * LD;
     const int g0 = blk_g0(M, tab, p), w = blk_rows(M, tab, p);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int tp = tab.blk_tile[p];
@@ -249,6 +263,15 @@ __global__ __launch_bounds__(NTHREADS) void k_residual(const DevMat* Mp, SweepTa
             atomicMax(norms + 3 * n + 1, (unsigned long long)__double_as_longlong(fabs(X[gi + (long)n * ldx])));
             atomicMax(norms + 3 * n + 2, (unsigned long long)__double_as_longlong(fabs(b)));
         }
+}
+
+template <int NR>
+__global__ __launch_bounds__(NTHREADS) void k_residual(const DevMat* Mp, SweepTab tab, const double* B, long ldb,
+                                                       const double* X, long ldx, double* R, long ldr, int nrhs,
+                                                       unsigned long long* norms, int rownorm) {
+    __shared__ __align__(16) char lds[sweep_lds_bytes<double, NR>()];
+    if ((int)blockIdx.x >= tab.P) return;
+    item_resid<NR>(*Mp, tab, blockIdx.x, B, ldb, X, ldx, R, ldr, nrhs, norms, rownorm, lds);
 }
 
 }  // namespace ttad
