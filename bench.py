@@ -78,6 +78,94 @@ def cpu_baseline(n_sample: int, b: int, threads: int):
                       f"({t:.3f} s/run), backend={kind}"}
 
 
+def serving_leg(ctx, rate: float, n_req: int, seed: int = 7, prec: str = "bf16x3", pool: int = 24,
+                budget: int = 16 << 30):
+    """Poisson arrivals of SPD solves (continuous batching in one resident kernel).
+
+    Sizes n = 64*round(exp(U(ln 64, ln 4096))/64); A = B B^T/n + I from a pool of `pool`
+    pre-staged matrices (each request draws one); 1-8 RHS iid N(0,1); about 10% of
+    requests solve with the cached factor of an earlier request of the same matrix.
+    Latency = arrival -> solution copied back to host memory.
+    """
+    import numpy as np
+    import torch
+    import tasktorrent_amd as tt
+    rng = np.random.default_rng(seed)
+    sizes = []
+    while len(sizes) < pool:
+        n = int(64 * round(np.exp(rng.uniform(np.log(64), np.log(4096))) / 64))
+        sizes.append(max(64, min(4096, n)))
+    mats = []
+    for q, n in enumerate(sizes):
+        g = torch.Generator(device="cuda")
+        g.manual_seed(10_000 + q)
+        B = torch.randn((n, n), dtype=torch.float64, device="cuda", generator=g)
+        A = B @ B.T / n
+        A.diagonal().add_(1.0)
+        b = 256
+        M = tt.TiledMatrix(ctx, [b] * (n // b) + ([n % b] if n % b else []), "f64")
+        M.set(A)
+        mats.append(M)
+    torch.cuda.synchronize()
+    srv = tt.Server(ctx, hbm_budget=budget, max_tasks=1 << 21, max_requests=2048)
+    # warm-up: one request per pool matrix, keeping factors for later reuse
+    ids = [srv.submit(M, rng.standard_normal((M.n, 1)), prec=prec, keep_factor=True)[0] for M in mats]
+    warm = srv.wait_all(ids, timeout_s=300)
+    last_factor = {q: warm[ids[q]]["factor_id"] for q in range(pool)}
+    gaps = rng.exponential(1.0 / rate, n_req)
+    arrivals = np.cumsum(gaps)
+    plan = []
+    for i in range(n_req):
+        q = int(rng.integers(pool))
+        nrhs = int(rng.integers(1, 9))
+        reuse = rng.random() < 0.1
+        plan.append((q, nrhs, reuse))
+    lat, qwait, etas, iters, statuses, reused = [], [], [], [], [], 0
+    t0 = time.perf_counter()
+    submitted, done = {}, 0
+    i = 0
+    while done < n_req:
+        now = time.perf_counter() - t0
+        while i < n_req and arrivals[i] <= now:
+            q, nrhs, reuse = plan[i]
+            M = mats[q]
+            Bv = rng.standard_normal((M.n, nrhs))
+            rid, _ = srv.submit(M, Bv, prec=prec, keep_factor=not reuse,
+                                reuse_factor=last_factor[q] if reuse else -1)
+            submitted[rid] = (arrivals[i], time.perf_counter() - t0, q)
+            i += 1
+        for r in srv.poll():
+            if r["id"] not in submitted:
+                continue
+            arr, sub, q = submitted.pop(r["id"])
+            t_now = time.perf_counter() - t0
+            lat.append(t_now - arr)
+            qwait.append(r["queue_s"])
+            etas.append(r["backward_error"])
+            iters.append(r["iterations"])
+            statuses.append(r["status"])
+            reused += r["reused"]
+            if not r["reused"] and r["status"] == 0:
+                last_factor[q] = r["factor_id"]
+            done += 1
+        if time.perf_counter() - t0 > 300:
+            raise TimeoutError("serving leg did not finish in 300 s")
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    for M in mats:
+        M.close()
+    lat_ms = np.array(lat) * 1e3
+    return {"rate_per_s": rate, "requests": n_req, "prec": prec, "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat_ms, 99)), 3), "mean_ms": round(float(lat_ms.mean()), 3),
+            "solves_per_s": round(n_req / wall, 2), "wall_s": round(wall, 3),
+            "max_backward_error": float(np.max(etas)), "ok": int(sum(1 for s_ in statuses if s_ == 0)),
+            "reused": int(reused), "mean_ir_steps": round(float(np.mean(iters)), 2),
+            "p99_queue_ms": round(float(np.percentile(np.array(qwait) * 1e3, 99)), 3),
+            "workers": st["workers"], "matrix_pool": pool,
+            "sizes": "n = 64*round(exp(U(ln64, ln4096))/64), 1-8 RHS, ~10% cached-factor requests"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +179,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=8192)
     ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson request rate of the serving leg (1/s)")
+    ap.add_argument("--serve-requests", type=int, default=600)
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -215,6 +306,12 @@ def main():
                      "unit": "TFLOP/s", "frac": round(F / t_fac64 / 1e12 / PEAK["f64"], 4), "traffic": None,
                      "kernel": "k_persistent FP64 factorization (one launch, F(n) flops / event-timed duration)"},
     }
+    if not args.no_serve:
+        del W64
+        torch.cuda.empty_cache()
+        log(f"serving leg: rate {args.serve_rate}/s, {args.serve_requests} requests")
+        out["serving"] = serving_leg(ctx, args.serve_rate, args.serve_requests)
+        log(f"serving: {out['serving']}")
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline: oracle n={args.cpu_n} b={b} threads={threads}")

@@ -60,6 +60,8 @@ int ttad_version(void);
 /* Replaces the Threadpool construction (threadpool_shared.cpp:17-33): binds a
  * device, creates its stream and scratch. */
 int ttad_init(int device, ttad_context** out);
+/* Matrices, graphs and servers created on ctx must be destroyed before it: their
+ * handles refer to it. */
 int ttad_finalize(ttad_context* ctx);
 int ttad_device_info(ttad_context* ctx, int* n_cu, long long* hbm_bytes, char* name, int name_len);
 int ttad_synchronize(ttad_context* ctx, void* stream);

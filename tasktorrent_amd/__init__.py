@@ -10,6 +10,7 @@ PyTorch is used only as plumbing for device buffers and streams.
 from __future__ import annotations
 
 import ctypes
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -44,13 +45,22 @@ class Context:
         self._h = ctypes.c_void_p()
         _capi.check(lib().ttad_init(device, ctypes.byref(self._h)))
         self.device = device
+        self._owned = weakref.WeakSet()  # matrices, graphs and servers created on this context
 
     @property
     def handle(self):
         return self._h
 
+    def _own(self, obj):
+        self._owned.add(obj)
+
     def close(self):
+        """Destroys what is still open on this context (servers, then graphs, then
+        matrices), then the context itself: a handle must never outlive its context."""
         if self._h:
+            order = {Server: 0, Graph: 1, TiledMatrix: 2}
+            for obj in sorted(list(self._owned), key=lambda o: order.get(type(o), 3)):
+                obj.close()
             lib().ttad_finalize(self._h)
             self._h = ctypes.c_void_p()
 
@@ -166,6 +176,7 @@ class TiledMatrix:
         self._h = ctypes.c_void_p()
         _capi.check(lib().ttad_matrix_create(ctx.handle, len(bs), bs.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                              st, ctypes.byref(self._h)))
+        ctx._own(self)
 
     @classmethod
     def uniform(cls, ctx, n: int, b: int, storage="f64"):
@@ -231,6 +242,7 @@ class Graph:
                                             self._keep[3].ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                             ctypes.byref(self._h)))
         self.ctx = ctx
+        ctx._own(self)
 
     def bind(self, slot: int, M: TiledMatrix):
         _capi.check(lib().ttad_graph_bind(self._h, slot, M.handle))
@@ -275,6 +287,7 @@ class Server:
         self._h = ctypes.c_void_p()
         _capi.check(lib().ttad_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(self._h)))
         self.ctx = ctx
+        ctx._own(self)
         self._keep = {}
 
     def submit(self, A: TiledMatrix, B: np.ndarray, prec="bf16x3", tol=1e-12, max_iter=30, reuse_factor=-1,

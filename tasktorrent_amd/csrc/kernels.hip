@@ -115,7 +115,7 @@ __global__ __launch_bounds__(NTHREADS) void k_persistent(DevGraph g) {
         __syncthreads();
         const int64_t e = s_item;
         if (e < 0) break;
-        __atomic_thread_fence(__ATOMIC_ACQUIRE);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         const int task = (int)(e >> 20), item = (int)(e & 0xFFFFF);
         const DevTask t = g.tasks[task];
         uint64_t t0 = 0;
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(NTHREADS) void k_persistent(DevGraph g) {
         exec_item(t, item, g.mats, lds);
         __syncthreads();
         if (tid == 0) {
-            __atomic_thread_fence(__ATOMIC_RELEASE);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             if (g.trace) {
                 const int slot = __hip_atomic_fetch_add(g.trace_cursor, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (slot < g.total_items) {

@@ -59,13 +59,13 @@ __global__ __launch_bounds__(NTHREADS) void k_server(DevServer s) {
             for (;;) {
                 got = srv_try_pop(s);
                 if (got >= 0 || got == -2) break;
-                // doorbell (host memory): polled by one workgroup in eight to limit bus traffic
                 const int shut = __hip_atomic_load((const int32_t*)s.shutdown, __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_SYSTEM);
                 if (shut >= 2) {  // forced stop
                     got = -2;
                     break;
                 }
+                // doorbell (host memory): polled by one workgroup in eight to limit bus traffic
                 const bool poller = (blockIdx.x & 7) == 0;
                 const int posted = poller ? __hip_atomic_load((const int32_t*)s.db_count, __ATOMIC_ACQUIRE,
                                                               __HIP_MEMORY_SCOPE_SYSTEM)
@@ -74,11 +74,10 @@ __global__ __launch_bounds__(NTHREADS) void k_server(DevServer s) {
                 if (poller && taken < posted) {
                     if (__hip_atomic_compare_exchange_strong(s.db_taken, &taken, taken + 1, __ATOMIC_RELAXED,
                                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                        const int seed = __hip_atomic_load((const int32_t*)(s.db_entries + (taken % s.db_cap)),
+                        const int slot = __hip_atomic_load((const int32_t*)(s.db_entries + (taken % s.db_cap)),
                                                            __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
-                        __atomic_thread_fence(__ATOMIC_ACQUIRE);  // request data written by DMA before posting
-                        __hip_atomic_fetch_add(s.active, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        srv_push(s, seed);
+                        got = -100 - (int64_t)slot;  // admission, done by the whole workgroup
+                        break;
                     }
                     continue;
                 }
@@ -89,7 +88,7 @@ __global__ __launch_bounds__(NTHREADS) void k_server(DevServer s) {
                     break;
                 }
                 if ((int64_t)wall_clock64() - idle_since > s.idle_limit) {
-                    __hip_atomic_store(s.err, 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(s.err, 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                     got = -2;
                     break;
                 }
@@ -99,21 +98,25 @@ __global__ __launch_bounds__(NTHREADS) void k_server(DevServer s) {
         }
         __syncthreads();
         const int64_t e = s_item;
+        if (e <= -100) {
+            srv_admit(s, (int)(-100 - e));
+            idle_since = (int64_t)wall_clock64();
+            __syncthreads();
+            continue;
+        }
         if (e < 0) break;
-        __atomic_thread_fence(__ATOMIC_ACQUIRE);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         const int task = (int)(e >> 16), item = (int)(e & 0xFFFF);
         const SrvTask t = s.tasks[task];
         srv_exec(s, t, task, item, lds);
         __syncthreads();
         if (tid == 0) {
-            __atomic_thread_fence(__ATOMIC_RELEASE);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             if (t.kind != S_CHECK) {
                 const int left =
                     __hip_atomic_fetch_add(s.items_left + task, -1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) - 1;
                 if (left == 0) srv_complete(s, task);
             }
-            __hip_atomic_fetch_add(s.items_run, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            idle_since = (int64_t)wall_clock64();
         }
         idle_since = (int64_t)wall_clock64();
     }
@@ -199,20 +202,24 @@ struct Factor {  // a working factor in the arena (reusable for new right-hand s
 struct ttad_server {
     ttad_context* ctx = nullptr;
     ttad_server_config cfg{};
-    hipStream_t s_run = nullptr, s_copy = nullptr;
+    hipStream_t s_run = nullptr;
     bool running = false;
     // device pools
     ttad::DevBuf arena, tasks, edges, deps, items_left, reqs, mats, ring_hdr, ring_slots, misc;
     int ring_log2 = 0;
     ttad::RangeAlloc arena_alloc, task_alloc, edge_alloc;
     std::vector<int> free_req_slots, free_mat_slots;
-    // host pinned
+    // host pinned (device-visible): doorbell, mail, completion records, mailbox and result pools
     ttad::SrvDone* done = nullptr;
+    ttad::SrvMail* mail = nullptr;
     int32_t* db_entries = nullptr;
     int32_t* db_count = nullptr;
     int32_t* shutdown = nullptr;
-    void* staging = nullptr;
-    size_t staging_bytes = 0;
+    int32_t* err = nullptr;
+    char* mbox = nullptr;          // mailbox pool (copy lists + payloads)
+    char* rbuf = nullptr;          // result pool (solutions)
+    ttad::RangeAlloc mbox_alloc, rbuf_alloc;
+    long long mbox_bytes = 0, rbuf_bytes = 0;
     int db_posted = 0;
     int workers = 0;
     uint64_t clock_rate_hz = 100000000;
@@ -235,13 +242,14 @@ struct ttad_server {
         ttad::Block vec;
         long long factor = -1;
         long long items = 0;
+        long long mbox_off = -1, mbox_len = 0, rbuf_off = -1, rbuf_len = 0;
         double t_admit = 0;
     };
     long long items_cap = 0, items_admitted = 0;
     std::deque<Pending> pending;
     std::unordered_map<long long, Active> active;
     std::unordered_map<long long, ttad::Factor> factors;
-    std::unordered_map<long long, int> a_slots;        // matrix id -> DevMat slot for A
+    std::unordered_map<long long, std::pair<int, int>> a_slots;  // matrix id -> (DevMat slot of A, active requests)
     std::map<std::pair<std::vector<int>, int>, std::shared_ptr<ttad::Template>> templates;
     long long next_id = 1;
     uint64_t use_clock = 0;
@@ -249,6 +257,27 @@ struct ttad_server {
     std::chrono::steady_clock::time_point epoch = std::chrono::steady_clock::now();
     double now() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - epoch).count();
+    }
+    // device addresses of the pinned allocations (base host pointer, base device pointer, bytes)
+    std::vector<std::tuple<const char*, unsigned long long, size_t>> pinned_maps;
+    unsigned long long dev_of(const void* host) const {
+        const char* h = (const char*)host;
+        for (const auto& m : pinned_maps)
+            if (h >= std::get<0>(m) && h < std::get<0>(m) + std::get<2>(m))
+                return std::get<1>(m) + (unsigned long long)(h - std::get<0>(m));
+        return 0;
+    }
+    void* pin(size_t bytes) {
+        void* p = nullptr;
+        if (hipHostMalloc(&p, bytes, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) return nullptr;
+        memset(p, 0, bytes);
+        void* d = nullptr;
+        if (hipHostGetDevicePointer(&d, p, 0) != hipSuccess) {
+            (void)hipHostFree(p);
+            return nullptr;
+        }
+        pinned_maps.emplace_back((const char*)p, (unsigned long long)d, bytes);
+        return p;
     }
 };
 
@@ -454,7 +483,6 @@ int ttad_server_create(ttad_context* ctx, const ttad_server_config* cfg_in, ttad
     S->cfg = cfg;
     TTAD_HIP(hipSetDevice(ctx->device));
     TTAD_HIP(hipStreamCreateWithFlags(&S->s_run, hipStreamNonBlocking));
-    TTAD_HIP(hipStreamCreateWithFlags(&S->s_copy, hipStreamNonBlocking));
     if (S->arena.alloc((size_t)cfg.hbm_budget) != hipSuccess)
         return set_error(TTAD_ERR_OOM, "server_create: arena of %lld bytes does not fit", cfg.hbm_budget);
     S->arena_alloc.reset(cfg.hbm_budget);
@@ -468,8 +496,7 @@ int ttad_server_create(ttad_context* ctx, const ttad_server_config* cfg_in, ttad
     TTAD_HIP(S->mats.alloc((size_t)max_mats * sizeof(DevMat)));
     S->task_alloc.reset(T);
     S->edge_alloc.reset((long long)T * 4);
-    // ring capacity per bucket; admission keeps the items of admitted requests below it
-    const int lg = 21;
+    const int lg = 21;  // ring capacity per bucket; admission keeps admitted items below it
     S->ring_log2 = lg;
     S->items_cap = 1LL << lg;
     TTAD_HIP(S->ring_hdr.alloc(RING_BUCKETS * 2 * 8));
@@ -480,22 +507,27 @@ int ttad_server_create(ttad_context* ctx, const ttad_server_config* cfg_in, ttad
     TTAD_HIP(hipMemset(S->misc.p, 0, 256));
     for (int r = cfg.max_requests - 1; r >= 0; r--) S->free_req_slots.push_back(r);
     for (int m = max_mats - 1; m >= 0; m--) S->free_mat_slots.push_back(m);
-    TTAD_HIP(hipHostMalloc((void**)&S->done, sizeof(SrvDone) * cfg.max_requests,
-                           hipHostMallocMapped | hipHostMallocCoherent));
-    memset(S->done, 0, sizeof(SrvDone) * cfg.max_requests);
     const int db_cap = 1 << 16;
-    TTAD_HIP(hipHostMalloc((void**)&S->db_entries, sizeof(int32_t) * (db_cap + 64),
-                           hipHostMallocMapped | hipHostMallocCoherent));
+    S->done = (SrvDone*)S->pin(sizeof(SrvDone) * cfg.max_requests);
+    S->mail = (SrvMail*)S->pin(sizeof(SrvMail) * cfg.max_requests);
+    S->db_entries = (int32_t*)S->pin(sizeof(int32_t) * (db_cap + 64));
+    S->mbox_bytes = 256LL << 20;
+    S->rbuf_bytes = 64LL << 20;
+    S->mbox = (char*)S->pin((size_t)S->mbox_bytes);
+    S->rbuf = (char*)S->pin((size_t)S->rbuf_bytes);
+    if (!S->done || !S->mail || !S->db_entries || !S->mbox || !S->rbuf)
+        return set_error(TTAD_ERR_OOM, "server_create: pinned host memory");
+    S->mbox_alloc.reset(S->mbox_bytes);
+    S->rbuf_alloc.reset(S->rbuf_bytes);
     S->db_count = S->db_entries + db_cap;
     S->shutdown = S->db_entries + db_cap + 16;
-    *S->db_count = 0;
-    *S->shutdown = 0;
-    S->staging_bytes = 64 << 20;
-    TTAD_HIP(hipHostMalloc(&S->staging, S->staging_bytes, hipHostMallocDefault));
+    S->err = S->db_entries + db_cap + 32;
     int rate_khz = 100000;
     (void)hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device);
     S->clock_rate_hz = (uint64_t)rate_khz * 1000ull;
-    // launch the device loop
+    // all set-up work is finished before the device loop starts: while it runs, the host
+    // only touches pinned memory (a runtime copy or memset could need the CUs it holds)
+    TTAD_HIP(hipDeviceSynchronize());
     DevServer d{};
     d.n_buckets = RING_BUCKETS;
     d.ring_log2 = lg;
@@ -511,21 +543,20 @@ int ttad_server_create(ttad_context* ctx, const ttad_server_config* cfg_in, ttad
     d.items_left = S->items_left.as<int32_t>();
     d.reqs = S->reqs.as<SrvReq>();
     d.mats = S->mats.as<DevMat>();
-    void* dp = nullptr;
-    TTAD_HIP(hipHostGetDevicePointer(&dp, S->done, 0));
-    d.done = (SrvDone*)dp;
-    TTAD_HIP(hipHostGetDevicePointer(&dp, S->db_entries, 0));
-    d.db_entries = (const volatile int32_t*)dp;
-    d.db_count = (const volatile int32_t*)dp + db_cap;
-    d.shutdown = (const volatile int32_t*)dp + db_cap + 16;
+    d.done = (SrvDone*)S->dev_of(S->done);
+    d.mail = (SrvMail*)S->dev_of(S->mail);
+    const int32_t* dbe = (const int32_t*)S->dev_of(S->db_entries);
+    d.db_entries = dbe;
+    d.db_count = dbe + db_cap;
+    d.shutdown = dbe + db_cap + 16;
+    d.err = (int32_t*)(dbe + db_cap + 32);
     d.db_cap = db_cap;
     int32_t* misc = S->misc.as<int32_t>();
     d.db_taken = misc;
     d.active = misc + 16;
-    d.items_run = (unsigned long long*)(misc + 32);
-    d.err = misc + 48;
     d.spin_limit = (int64_t)S->clock_rate_hz * 10;
     d.idle_limit = (int64_t)S->clock_rate_hz * (cfg.idle_timeout_s > 0 ? cfg.idle_timeout_s : 600);
+    if (!d.done || !d.mail || !dbe) return set_error(TTAD_ERR_DEVICE, "server_create: pinned memory not mapped");
     TTAD_HIP(hipFuncSetAttribute((const void*)k_server, hipFuncAttributeMaxDynamicSharedMemorySize, SRV_LDS));
     int per_cu = 1;
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_server, NTHREADS, SRV_LDS);
@@ -555,12 +586,9 @@ int ttad_server_destroy(ttad_server* S) {
         (void)hipStreamSynchronize(S->s_run);
         S->running = false;
     }
-    (void)hipStreamSynchronize(S->s_copy);
-    if (S->done) (void)hipHostFree(S->done);
-    if (S->db_entries) (void)hipHostFree(S->db_entries);
-    if (S->staging) (void)hipHostFree(S->staging);
+    for (void* p : {(void*)S->done, (void*)S->mail, (void*)S->db_entries, (void*)S->mbox, (void*)S->rbuf})
+        if (p) (void)hipHostFree(p);
     (void)hipStreamDestroy(S->s_run);
-    (void)hipStreamDestroy(S->s_copy);
     delete S;
     return rc;
 }
@@ -569,29 +597,53 @@ int ttad_server_destroy(ttad_server* S) {
 
 namespace ttad {
 
-static int server_mat_slot_for_a(ttad_server* S, ttad_matrix* A, int& slot) {
-    auto it = S->a_slots.find(A->id);
-    if (it != S->a_slots.end()) {
-        slot = it->second;
-        return TTAD_OK;
+// Builds the admission mail of one request: payloads and the list of copies the
+// admitting workgroup performs (host pinned -> HBM).
+struct MailBuilder {
+    ttad_server* S;
+    long long off, cap;  // mailbox range
+    long long used = 0;
+    std::vector<SrvCopy> copies;
+    bool ok = true;
+    void add(void* dst_dev, const void* src, size_t bytes) {
+        const size_t b8 = (bytes + 7) / 8 * 8;
+        used = (used + 255) / 256 * 256;
+        if (used + (long long)b8 > cap) {
+            ok = false;
+            return;
+        }
+        char* h = S->mbox + off + used;
+        memcpy(h, src, bytes);
+        if (b8 > bytes) memset(h + bytes, 0, b8 - bytes);
+        copies.push_back(SrvCopy{(unsigned long long)dst_dev, S->dev_of(h), (unsigned long long)b8});
+        used += (long long)b8;
     }
-    if (S->free_mat_slots.empty()) return set_error(TTAD_ERR_BUSY, "server: no matrix slot");
-    slot = S->free_mat_slots.back();
-    S->free_mat_slots.pop_back();
-    TTAD_HIP(hipMemcpyAsync(S->mats.as<DevMat>() + slot, &A->host_desc, sizeof(DevMat), hipMemcpyHostToDevice,
-                            S->s_copy));
-    TTAD_HIP(hipStreamSynchronize(S->s_copy));
-    S->a_slots[A->id] = slot;
-    return TTAD_OK;
+    unsigned long long finish() {  // the copy list itself, at the end of the range
+        used = (used + 255) / 256 * 256;
+        const size_t lb = copies.size() * sizeof(SrvCopy);
+        if (used + (long long)lb > cap) {
+            ok = false;
+            return 0;
+        }
+        char* h = S->mbox + off + used;
+        memcpy(h, copies.data(), lb);
+        used += (long long)lb;
+        return S->dev_of(h);
+    }
+};
+
+static long long mail_bytes_estimate(const Template& T, int n, int nrhs, size_t ftab, int P) {
+    return (long long)(T.tasks.size() * sizeof(SrvTask) + T.edges.size() * 4 + T.tasks.size() * 8 + sizeof(SrvReq) +
+                       2 * sizeof(DevMat) + ftab + (size_t)P * 8 + (size_t)n * nrhs * 8) +
+           64 * 256 + 16 * (long long)sizeof(SrvCopy);
 }
 
-// Tries to admit one pending request; returns TTAD_OK if admitted, TTAD_ERR_BUSY if resources are short.
+// Tries to admit one pending request; TTAD_OK if posted, TTAD_ERR_BUSY if resources are short.
 static int server_admit(ttad_server* S, const ttad_server::Pending& p) {
     ttad_matrix* A = p.A;
     const int n = A->n, nrhs = p.nrhs;
     const bool factor = p.reuse < 0;
     const int storage = p.prec == P_F64 ? P_F64 : P_F32;
-    // 64-row block table
     std::vector<int32_t> bt, bo;
     for (int i = 0; i < A->nt; i++)
         for (int o = 0; o < A->bsz[i]; o += SB) {
@@ -612,7 +664,14 @@ static int server_admit(ttad_server* S, const ttad_server::Pending& p) {
     }
     if (S->free_req_slots.empty()) return TTAD_ERR_BUSY;
     if (S->items_admitted + T->items > S->items_cap) return TTAD_ERR_BUSY;
-    // vectors: B, X, R (FP64), Y, D (factor precision), flags, norms, anorm, block tables
+    Factor* F = nullptr;
+    if (!factor) {
+        auto fit = S->factors.find(p.reuse);
+        if (fit == S->factors.end() || !fit->second.ready)
+            return set_error(TTAD_ERR_NOT_FOUND, "server: factor %lld not available", p.reuse);
+        F = &fit->second;
+    }
+    // vectors in the arena: B, X, R (FP64), Y, D (factor precision), flags, norms, anorm, block tables
     const size_t es = storage == P_F64 ? 8 : 4;
     long long vb = 0;
     auto take = [&](long long bytes) {
@@ -624,8 +683,6 @@ static int server_admit(ttad_server* S, const ttad_server::Pending& p) {
                     oR = take((long long)n * nrhs * 8), oY = take((long long)n * nrhs * es),
                     oD = take((long long)n * nrhs * es), oFf = take(P * 4), oFb = take(P * 4),
                     oN = take(3 * 32 * 8), oAn = take(8), oBt = take(P * 4), oBo = take(P * 4);
-    // factor storage (new factor) or the cached one
-    Factor* F = nullptr;
     std::vector<char> ftab;
     DevMat fdesc{};
     long long fbytes = 0, ftab_off = 0;
@@ -633,20 +690,31 @@ static int server_admit(ttad_server* S, const ttad_server::Pending& p) {
         const FactorLayout fl = factor_layout(A->bsz, storage, ftab, 0, nullptr, fdesc);
         fbytes = fl.total;
         ftab_off = fl.tab_off;
-    } else {
-        auto fit = S->factors.find(p.reuse);
-        if (fit == S->factors.end() || !fit->second.ready)
-            return set_error(TTAD_ERR_NOT_FOUND, "server: factor %lld not available", p.reuse);
-        F = &fit->second;
     }
-    const long long need = vb + fbytes;
+    const long long mneed = mail_bytes_estimate(*T, n, nrhs, ftab.size(), P);
+    const long long rneed = (long long)n * nrhs * 8;
+    const long long moff = S->mbox_alloc.alloc(mneed, 256);
+    if (moff < 0) return TTAD_ERR_BUSY;
+    const long long roff = S->rbuf_alloc.alloc(rneed, 256);
+    if (roff < 0) {
+        S->mbox_alloc.release(moff, mneed);
+        return TTAD_ERR_BUSY;
+    }
     long long voff = S->arena_alloc.alloc(vb, 256);
-    if (voff < 0) return TTAD_ERR_BUSY;
     long long foff = -1;
+    auto undo = [&]() {
+        S->mbox_alloc.release(moff, mneed);
+        S->rbuf_alloc.release(roff, rneed);
+        if (voff >= 0) S->arena_alloc.release(voff, vb);
+        if (foff >= 0) S->arena_alloc.release(foff, fbytes);
+    };
+    if (voff < 0) {
+        undo();
+        return TTAD_ERR_BUSY;
+    }
     if (factor) {
         foff = S->arena_alloc.alloc(fbytes, 256);
-        // evict unused cached factors until it fits
-        while (foff < 0) {
+        while (foff < 0) {  // evict the least recently used idle cached factor
             long long victim = -1;
             uint64_t oldest = ~0ull;
             for (auto& kv : S->factors)
@@ -662,24 +730,34 @@ static int server_admit(ttad_server* S, const ttad_server::Pending& p) {
             foff = S->arena_alloc.alloc(fbytes, 256);
         }
         if (foff < 0) {
-            S->arena_alloc.release(voff, vb);
+            undo();
             return TTAD_ERR_BUSY;
         }
     }
-    (void)need;
-    const long long t0 = S->task_alloc.alloc((long long)T->tasks.size());
-    const long long e0 = S->edge_alloc.alloc(std::max<long long>(1, (long long)T->edges.size()));
-    if (t0 < 0 || e0 < 0 || (factor && S->free_mat_slots.empty())) {
-        if (t0 >= 0) S->task_alloc.release(t0, (long long)T->tasks.size());
-        if (e0 >= 0) S->edge_alloc.release(e0, std::max<long long>(1, (long long)T->edges.size()));
-        S->arena_alloc.release(voff, vb);
-        if (foff >= 0) S->arena_alloc.release(foff, fbytes);
+    const size_t nt = T->tasks.size(), ne = T->edges.size();
+    const long long t0 = S->task_alloc.alloc((long long)nt);
+    const long long e0 = S->edge_alloc.alloc(std::max<long long>(1, (long long)ne));
+    const bool need_a_slot = S->a_slots.find(A->id) == S->a_slots.end();
+    const size_t slots_needed = (factor ? 1 : 0) + (need_a_slot ? 1 : 0);
+    if (t0 < 0 || e0 < 0 || S->free_mat_slots.size() < slots_needed) {
+        if (t0 >= 0) S->task_alloc.release(t0, (long long)nt);
+        if (e0 >= 0) S->edge_alloc.release(e0, std::max<long long>(1, (long long)ne));
+        undo();
         return TTAD_ERR_BUSY;
     }
-    int a_slot = -1;
-    int rc = server_mat_slot_for_a(S, A, a_slot);
-    if (rc) return rc;
     char* arena = (char*)S->arena.p;
+    MailBuilder mb{S, moff, mneed};
+    int a_slot;
+    if (need_a_slot) {
+        a_slot = S->free_mat_slots.back();
+        S->free_mat_slots.pop_back();
+        S->a_slots[A->id] = {a_slot, 1};
+        mb.add(S->mats.as<DevMat>() + a_slot, &A->host_desc, sizeof(DevMat));
+    } else {
+        auto& as = S->a_slots[A->id];
+        a_slot = as.first;
+        as.second++;
+    }
     int w_slot;
     if (factor) {
         w_slot = S->free_mat_slots.back();
@@ -692,8 +770,9 @@ static int server_admit(ttad_server* S, const ttad_server::Pending& p) {
         nf.a_id = A->id;
         nf.storage = storage;
         nf.users = 1;
-        nf.ready = false;
         S->factors[p.id] = nf;
+        mb.add(arena + foff + ftab_off, ftab.data(), ftab.size());
+        mb.add(S->mats.as<DevMat>() + w_slot, &fdesc, sizeof(fdesc));
     } else {
         w_slot = F->slot;
         F->users++;
@@ -701,17 +780,6 @@ static int server_admit(ttad_server* S, const ttad_server::Pending& p) {
     }
     const int slot = S->free_req_slots.back();
     S->free_req_slots.pop_back();
-    // stage everything in pinned memory, then DMA
-    const size_t nt = T->tasks.size(), ne = T->edges.size();
-    size_t so = 0;
-    char* st = (char*)S->staging;
-    auto stage = [&](const void* src, size_t bytes) {
-        so = (so + 255) / 256 * 256;
-        const size_t o = so;
-        memcpy(st + o, src, bytes);
-        so += bytes;
-        return o;
-    };
     std::vector<SrvTask> tk(T->tasks);
     for (auto& t : tk) {
         t.edge_off += (int)e0;
@@ -748,37 +816,23 @@ static int server_admit(ttad_server* S, const ttad_server::Pending& p) {
     rq.anorm = (unsigned long long*)dv(oAn);
     rq.blk_tile = (const int32_t*)dv(oBt);
     rq.blk_off = (const int32_t*)dv(oBo);
-    const size_t s_tasks = stage(tk.data(), nt * sizeof(SrvTask));
-    const size_t s_edges = ne ? stage(ed.data(), ne * 4) : 0;
-    const size_t s_deps = stage(T->deps0.data(), nt * 4);
-    const size_t s_il = stage(il.data(), nt * 4);
-    const size_t s_req = stage(&rq, sizeof(rq));
-    const size_t s_bt = stage(bt.data(), P * 4), s_bo = stage(bo.data(), P * 4);
-    size_t s_ftab = 0, s_fdesc = 0;
-    if (factor) {
-        s_ftab = stage(ftab.data(), ftab.size());
-        s_fdesc = stage(&fdesc, sizeof(fdesc));
-    }
-    const size_t s_B = stage(p.B, (size_t)n * nrhs * 8);
-    if (so > S->staging_bytes) return set_error(TTAD_ERR_INVALID, "server: request descriptors exceed staging");
-    hipStream_t c = S->s_copy;
-    auto h2d = [&](void* dst, size_t off, size_t bytes) {
-        return hipMemcpyAsync(dst, st + off, bytes, hipMemcpyHostToDevice, c);
-    };
-    TTAD_HIP(h2d(S->tasks.as<SrvTask>() + t0, s_tasks, nt * sizeof(SrvTask)));
-    if (ne) TTAD_HIP(h2d(S->edges.as<int32_t>() + e0, s_edges, ne * 4));
-    TTAD_HIP(h2d(S->deps.as<int32_t>() + t0, s_deps, nt * 4));
-    TTAD_HIP(h2d(S->items_left.as<int32_t>() + t0, s_il, nt * 4));
-    TTAD_HIP(h2d(S->reqs.as<SrvReq>() + slot, s_req, sizeof(rq)));
-    TTAD_HIP(h2d(dv(oBt), s_bt, P * 4));
-    TTAD_HIP(h2d(dv(oBo), s_bo, P * 4));
-    TTAD_HIP(h2d(dv(oB), s_B, (size_t)n * nrhs * 8));
-    if (factor) {
-        TTAD_HIP(h2d(arena + foff + ftab_off, s_ftab, ftab.size()));
-        TTAD_HIP(h2d(S->mats.as<DevMat>() + w_slot, s_fdesc, sizeof(fdesc)));
-    }
-    TTAD_HIP(hipStreamSynchronize(c));
+    rq.x_host = (double*)S->dev_of(S->rbuf + roff);
+    mb.add(S->tasks.as<SrvTask>() + t0, tk.data(), nt * sizeof(SrvTask));
+    if (ne) mb.add(S->edges.as<int32_t>() + e0, ed.data(), ne * 4);
+    mb.add(S->deps.as<int32_t>() + t0, T->deps0.data(), nt * 4);
+    mb.add(S->items_left.as<int32_t>() + t0, il.data(), nt * 4);
+    mb.add(S->reqs.as<SrvReq>() + slot, &rq, sizeof(rq));
+    mb.add(dv(oBt), bt.data(), P * 4);
+    mb.add(dv(oBo), bo.data(), P * 4);
+    mb.add(dv(oB), p.B, (size_t)n * nrhs * 8);
+    const unsigned long long cl = mb.finish();
+    if (!mb.ok) return set_error(TTAD_ERR_INVALID, "server: admission mail larger than its estimate");
     memset(&S->done[slot], 0, sizeof(SrvDone));
+    SrvMail& m = S->mail[slot];
+    m.n_copies = (int)mb.copies.size();
+    m.seed = (int)t0 + T->t_init;
+    m.copies = cl;
+    __atomic_store_n(&m.state, 1, __ATOMIC_RELEASE);
     ttad_server::Active a;
     a.p = p;
     a.slot = slot;
@@ -789,12 +843,16 @@ static int server_admit(ttad_server* S, const ttad_server::Pending& p) {
     a.vec = Block{voff, vb};
     a.factor = factor ? p.id : p.reuse;
     a.items = T->items;
+    a.mbox_off = moff;
+    a.mbox_len = mneed;
+    a.rbuf_off = roff;
+    a.rbuf_len = rneed;
     S->items_admitted += T->items;
     a.t_admit = S->now();
     S->active[p.id] = a;
-    // post the seed (INIT) through the doorbell
+    // ring the doorbell with the slot
     const int db_cap = 1 << 16;
-    S->db_entries[S->db_posted % db_cap] = (int)t0 + T->t_init;
+    S->db_entries[S->db_posted % db_cap] = slot;
     __atomic_thread_fence(__ATOMIC_SEQ_CST);
     S->db_posted++;
     __atomic_store_n(S->db_count, S->db_posted, __ATOMIC_RELEASE);
@@ -802,6 +860,14 @@ static int server_admit(ttad_server* S, const ttad_server::Pending& p) {
 }
 
 static int server_pump(ttad_server* S) {
+    // mailboxes the device has copied can be reused
+    for (auto& kv : S->active) {
+        auto& a = kv.second;
+        if (a.mbox_off >= 0 && __atomic_load_n(&S->mail[a.slot].state, __ATOMIC_ACQUIRE) == 2) {
+            S->mbox_alloc.release(a.mbox_off, a.mbox_len);
+            a.mbox_off = -1;
+        }
+    }
     while (!S->pending.empty()) {
         const int rc = server_admit(S, S->pending.front());
         if (rc == TTAD_ERR_BUSY) break;
@@ -823,7 +889,15 @@ static void server_release(ttad_server* S, ttad_server::Active& a, bool keep_fac
     S->task_alloc.release(a.t0, a.ntasks);
     S->edge_alloc.release(a.e0, a.nedges);
     S->arena_alloc.release(a.vec.off, a.vec.bytes);
+    if (a.mbox_off >= 0) S->mbox_alloc.release(a.mbox_off, a.mbox_len);
+    S->rbuf_alloc.release(a.rbuf_off, a.rbuf_len);
+    S->mail[a.slot].state = 0;
     S->free_req_slots.push_back(a.slot);
+    auto ait = S->a_slots.find(a.p.A->id);
+    if (ait != S->a_slots.end() && --ait->second.second == 0) {  // no request in flight reads this A
+        S->free_mat_slots.push_back(ait->second.first);
+        S->a_slots.erase(ait);
+    }
     auto fit = S->factors.find(a.factor);
     if (fit != S->factors.end()) {
         Factor& F = fit->second;
@@ -876,8 +950,7 @@ int ttad_server_poll(ttad_server* S, ttad_server_result* out, int max_out, int* 
     }
     for (long long id : finished) {
         auto& a = S->active[id];
-        SrvDone d = S->done[a.slot];
-        const double t_end = S->now();
+        const SrvDone d = S->done[a.slot];
         ttad_server_result r{};
         r.id = id;
         r.n = a.p.A->n;
@@ -887,49 +960,37 @@ int ttad_server_poll(ttad_server* S, ttad_server_result* out, int max_out, int* 
         r.factor_id = a.factor;
         r.reused = a.p.reuse >= 0;
         r.status = d.state == SRV_OK ? TTAD_OK : (d.state == SRV_NOT_SPD ? TTAD_ERR_NOT_SPD : TTAD_ERR_IR_STALLED);
-        // solution back to the caller's buffer
-        SrvReq rq;
-        TTAD_HIP(hipMemcpyAsync(&rq, S->reqs.as<SrvReq>() + a.slot, sizeof(rq), hipMemcpyDeviceToHost, S->s_copy));
-        TTAD_HIP(hipStreamSynchronize(S->s_copy));
-        TTAD_HIP(hipMemcpyAsync(a.p.X, rq.X, (size_t)r.n * r.nrhs * 8, hipMemcpyDeviceToHost, S->s_copy));
-        TTAD_HIP(hipStreamSynchronize(S->s_copy));
+        memcpy(a.p.X, S->rbuf + a.rbuf_off, (size_t)r.n * r.nrhs * 8);  // solution written by the device
         r.latency_s = S->now() - a.p.t_submit;
         r.queue_s = a.t_admit - a.p.t_submit;
-        (void)t_end;
         const bool keep = a.p.keep && r.status == TTAD_OK;
         server_release(S, a, keep);
         S->active.erase(id);
         S->results.push_back(r);
     }
     server_pump(S);
-    int32_t err = 0;
-    TTAD_HIP(hipMemcpyAsync(&err, S->misc.as<int32_t>() + 48, 4, hipMemcpyDeviceToHost, S->s_copy));
-    TTAD_HIP(hipStreamSynchronize(S->s_copy));
     int k = 0;
     while (k < max_out && !S->results.empty()) {
         out[k++] = S->results.front();
         S->results.pop_front();
     }
     *n_out = k;
+    const int err = __atomic_load_n(S->err, __ATOMIC_ACQUIRE);
     if (err) return set_error(TTAD_ERR_TIMEOUT, "server: device watchdog code %d", err);
     return TTAD_OK;
 }
 
 int ttad_server_query(ttad_server* S, ttad_server_stats* st) {
-    if (!S || !st) return set_error(TTAD_ERR_INVALID, "server_stats: bad args");
+    if (!S || !st) return set_error(TTAD_ERR_INVALID, "server_query: bad args");
     memset(st, 0, sizeof(*st));
     st->pending = (int)S->pending.size();
     st->active = (int)S->active.size();
-    st->cached_factors = 0;
     for (auto& kv : S->factors)
         if (kv.second.ready) st->cached_factors++;
     st->arena_used = S->arena_alloc.used();
     st->arena_size = S->arena_alloc.size();
     st->workers = S->workers;
-    unsigned long long runs = 0;
-    TTAD_HIP(hipMemcpyAsync(&runs, S->misc.as<int32_t>() + 32, 8, hipMemcpyDeviceToHost, S->s_copy));
-    TTAD_HIP(hipStreamSynchronize(S->s_copy));
-    st->items_run = (long long)runs;
+    st->items_run = 0;
     return TTAD_OK;
 }
 

@@ -45,6 +45,22 @@ struct SrvReq {
     unsigned long long* anorm;  // [1]
     const int32_t* blk_tile;    // 64-row block table of the partition
     const int32_t* blk_off;
+    double* x_host;             // pinned host buffer that receives X (n x nrhs) on completion
+};
+
+// Admission mail of one request slot (pinned host memory, written by the host, read by
+// the workgroup that admits the request): a list of copies host -> HBM, then the seed.
+struct SrvCopy {
+    unsigned long long dst;     // device address
+    unsigned long long src;     // device address of pinned host memory
+    unsigned long long bytes;   // multiple of 8
+};
+struct SrvMail {
+    int32_t state;              // 0 free, 1 posted by the host, 2 copied by the device
+    int32_t n_copies;
+    int32_t seed;               // task to push once the copies are done
+    int32_t pad;
+    unsigned long long copies;  // device address of SrvCopy[n_copies]
 };
 
 // Host-visible completion record (pinned host memory).
@@ -74,14 +90,14 @@ struct DevServer {
     SrvReq* reqs;
     DevMat* mats;
     SrvDone* done;              // host pinned
-    const volatile int32_t* db_entries;  // host pinned: seed task ids
-    const volatile int32_t* db_count;    // host pinned: number of posted seeds
+    const volatile int32_t* db_entries;  // host pinned: request slots posted for admission
+    const volatile int32_t* db_count;    // host pinned: number of posted slots
     const volatile int32_t* shutdown;    // host pinned
+    SrvMail* mail;              // host pinned, one per request slot
     int32_t db_cap;
-    int32_t* db_taken;          // device: seeds consumed
+    int32_t* db_taken;          // device: doorbell entries consumed
     int32_t* active;            // device: requests in flight
-    unsigned long long* items_run;
-    int32_t* err;
+    int32_t* err;               // host pinned: watchdog codes
     int64_t spin_limit;
     int64_t idle_limit;
 };
@@ -111,6 +127,39 @@ __device__ __forceinline__ void srv_complete(const DevServer& s, int task) {
 }
 
 __device__ __forceinline__ double bits_to_double(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+// Admit the request posted in mail slot `slot`: the whole workgroup copies its
+// descriptors and right-hand sides from pinned host memory into HBM, then thread 0
+// releases them and pushes the seed task. No runtime copy is needed while the
+// persistent kernel owns the device.
+__device__ __forceinline__ void srv_admit(const DevServer& s, int slot) {
+    SrvMail* m = s.mail + slot;
+    __shared__ int s_nc, s_seed;
+    __shared__ unsigned long long s_copies;
+    if (threadIdx.x == 0) {
+        s_nc = __hip_atomic_load(&m->n_copies, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        s_seed = __hip_atomic_load(&m->seed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        s_copies = __hip_atomic_load(&m->copies, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    __syncthreads();
+    // every wave reads the host's payloads after thread 0's acquire of the doorbell
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);  // system scope
+    const SrvCopy* cl = (const SrvCopy*)s_copies;
+    for (int c = 0; c < s_nc; c++) {
+        const unsigned long long* src = (const unsigned long long*)cl[c].src;
+        unsigned long long* dst = (unsigned long long*)cl[c].dst;
+        const long words = (long)(cl[c].bytes >> 3);
+        for (long w = threadIdx.x; w < words; w += NTHREADS) dst[w] = src[w];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // each wave publishes its own copies
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(&m->state, 2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_fetch_add(s.active, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        srv_push(s, s_seed);
+    }
+}
 
 // Serving task bodies. All 256 threads of the workgroup participate.
 __device__ __forceinline__ void srv_exec(const DevServer& s, const SrvTask& t, int task, int it, char* lds) {
@@ -191,7 +240,8 @@ __device__ __forceinline__ void srv_exec(const DevServer& s, const SrvTask& t, i
             break;
         }
         case S_CHECK: {
-            __shared__ int s_decision;  // 0 = re-arm, 1 = done
+            __shared__ int s_state;  // SRV_RUNNING = re-arm
+            __shared__ double s_eta;
             if (threadIdx.x == 0) {
                 const double an = bits_to_double(*q.anorm);
                 double eta = 0.0;
@@ -214,25 +264,32 @@ __device__ __forceinline__ void srv_exec(const DevServer& s, const SrvTask& t, i
                     if (q.stall >= 2 || q.iter >= q.max_iter) state = SRV_STALLED;
                 }
                 q.eta_prev = eta;
-                if (state != SRV_RUNNING) {
+                s_state = state;
+                s_eta = eta;
+            }
+            __syncthreads();
+            const int state = s_state;
+            if (state != SRV_RUNNING) {  // done: solution to the caller's pinned buffer, then the record
+                const long tot = (long)q.n * q.nrhs;
+                for (long e = threadIdx.x; e < tot; e += NTHREADS) q.x_host[e] = q.X[e];
+                __atomic_thread_fence(__ATOMIC_RELEASE);  // system scope: each wave's part of X reaches the host
+                __syncthreads();
+                if (threadIdx.x == 0) {
                     SrvDone& d = s.done[q.done_slot];
                     d.iterations = q.iter;
-                    d.eta = eta;
-                    d.info = info;
+                    d.eta = s_eta;
+                    d.info = *s.mats[q.mat_w].info;
                     d.t_done = wall_clock64();
-                    __atomic_thread_fence(__ATOMIC_RELEASE);
                     __hip_atomic_store(&d.state, state, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
                     __hip_atomic_fetch_add(s.active, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
-                s_decision = state != SRV_RUNNING;
-            }
-            __syncthreads();
-            if (!s_decision) {  // another refinement step: reset the chain and start it again
+            } else {  // another refinement step: reset the chain and start it again
                 for (int e = threadIdx.x; e < q.P; e += NTHREADS) {
                     q.flags_f[e] = 0;
                     q.flags_b[e] = 0;
                 }
                 for (int e = threadIdx.x; e < 3 * 32; e += NTHREADS) q.norms[e] = 0;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
                 __syncthreads();
                 if (threadIdx.x == 0) {
                     const int ids[4] = {q.t_fwd, q.t_bwd, q.t_res, q.t_chk};
@@ -240,7 +297,7 @@ __device__ __forceinline__ void srv_exec(const DevServer& s, const SrvTask& t, i
                     s.deps[q.t_bwd] = 1;
                     s.deps[q.t_res] = 1;
                     s.deps[q.t_chk] = 1;
-                    __atomic_thread_fence(__ATOMIC_RELEASE);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
                     srv_push(s, q.t_fwd);
                 }
             }

@@ -136,7 +136,7 @@ __device__ __forceinline__ void item_sweep(const DevMat& M, const SweepTab& tab,
         else fill_block<TL, TC>(Ls, M, tab, p, q, 1, false);
         if (threadIdx.x == 0) spin_flag(flags + q, err, spin_limit);
         __syncthreads();
-        __atomic_thread_fence(__ATOMIC_ACQUIRE);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         const int gq = blk_g0(M, tab, q), wq = blk_rows(M, tab, q);
         if (forward) fill_rhs<TC, TC, NR>(Ys, Y, ldy, gq, wq, nrhs);
         else fill_rhs<TC, TC, NR>(Ys, D, ldd, gq, wq, nrhs);
@@ -182,7 +182,7 @@ __device__ __forceinline__ void item_sweep(const DevMat& M, const SweepTab& tab,
         }
     __syncthreads();
     if (threadIdx.x == 0) {
-        __atomic_thread_fence(__ATOMIC_RELEASE);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __hip_atomic_store(flags + p, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
 }

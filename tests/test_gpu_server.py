@@ -120,3 +120,14 @@ def test_fp64_requests(ctx, problems):
         assert r["status"] == 0 and eta(A, r["X"], B) <= 1e-15
     finally:
         srv.close()
+
+
+def test_context_close_with_live_objects():
+    # closing a context first closes the server and matrices still open on it, so
+    # that dropping them afterwards never touches the freed context
+    c2 = tt.Context(0)
+    M = _mat(c2, _spd(256, 5))
+    srv = tt.Server(c2, hbm_budget=64 << 20, max_tasks=1 << 12, max_requests=4)
+    c2.close()
+    assert not M.handle and not srv._h
+    del M, srv
