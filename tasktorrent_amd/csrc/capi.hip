@@ -34,7 +34,8 @@ struct StreamPlan {
         int task_off, prefix_off, n_tasks, n_items;
     };
     std::vector<Batch> batches;
-    DevBuf d_tasks, d_prefix;
+    DevBuf d_tasks, d_prefix, d_err;
+    int vset = N_VSETS - 1;  // executor instantiation covering the plan's item variants
 };
 
 static std::shared_ptr<StreamPlan> build_stream_plan(ttad_matrix* M, int prec, int& rc) {
@@ -75,8 +76,10 @@ static std::shared_ptr<StreamPlan> build_stream_plan(ttad_matrix* M, int prec, i
         add_batch(up);
     }
     rc = TTAD_OK;
+    plan->vset = vset_index(item_variant(K_POTRF, prec, M->storage) | item_variant(K_GEMM, prec, M->storage));
     if (plan->d_tasks.alloc(std::max<size_t>(1, tasks.size()) * sizeof(DevTask)) != hipSuccess ||
-        plan->d_prefix.alloc(std::max<size_t>(1, prefix.size()) * sizeof(int32_t)) != hipSuccess) {
+        plan->d_prefix.alloc(std::max<size_t>(1, prefix.size()) * sizeof(int32_t)) != hipSuccess ||
+        plan->d_err.alloc(sizeof(int32_t)) != hipSuccess || hipMemset(plan->d_err.p, 0, sizeof(int32_t)) != hipSuccess) {
         rc = set_error(TTAD_ERR_OOM, "stream plan allocation failed");
         return nullptr;
     }
@@ -448,9 +451,14 @@ int ttad_potrf_tiled(ttad_context* ctx, ttad_matrix* M, int schedule, int prio_k
             plan = it->second;
         }
         for (auto& b : plan->batches)
-            launch_items(M->desc.as<DevMat>(), plan->d_tasks.as<DevTask>() + b.task_off,
-                         plan->d_prefix.as<int32_t>() + b.prefix_off, b.n_tasks, b.n_items, s);
+            launch_items(plan->vset, M->desc.as<DevMat>(), plan->d_tasks.as<DevTask>() + b.task_off,
+                         plan->d_prefix.as<int32_t>() + b.prefix_off, b.n_tasks, b.n_items, plan->d_err.as<int32_t>(),
+                         s);
         TTAD_HIP(hipGetLastError());
+        int err = 0;
+        TTAD_HIP(hipMemcpyAsync(&err, plan->d_err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        TTAD_HIP(hipStreamSynchronize(s));
+        if (err) return set_error(TTAD_ERR_DEVICE, "potrf_tiled: item variant missing from the executor (%d)", err);
     } else if (schedule == TTAD_SCHED_PERSISTENT) {
         rc = cholesky_persistent(ctx, M, prio_kind, prec, s, info);
         if (rc != TTAD_OK && rc != TTAD_ERR_NOT_SPD) return rc;
@@ -573,6 +581,9 @@ static int graph_prepare(ttad_graph* g) {
                 return set_error(TTAD_ERR_INVALID, "graph: task %d precision does not match tile storage", t);
             x.n_items = std::max(1, items_of(x.kind, M->bsz[x.i], M->bsz[x.j]));
         }
+        int needed = 0;
+        for (auto& x : g->tasks) needed |= item_variant(x.kind, x.prec, g->mats[x.mat]->storage);
+        g->vset = vset_index(needed);
         g->bucket_items.assign(g->n_buckets, 0);
         g->total_items = 0;
         for (auto& x : g->tasks) {
@@ -685,7 +696,7 @@ int ttad_graph_launch(ttad_graph* g, int n_workers, int trace, void* stream) {
     for (auto* M : g->mats)
         if (M) TTAD_HIP(hipMemsetAsync(M->info.p, 0, sizeof(int), g->stream));
     TTAD_HIP(hipEventRecord(g->ev_start, g->stream));
-    launch_persistent(d, g->workers, g->stream);
+    launch_persistent(g->vset, d, g->workers, g->stream);
     TTAD_HIP(hipGetLastError());
     TTAD_HIP(hipEventRecord(g->ev_stop, g->stream));
     g->running = true;

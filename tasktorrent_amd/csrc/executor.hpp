@@ -40,27 +40,70 @@ __device__ __forceinline__ int64_t ld_acquire64(const int64_t* p) {
     return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Execute work item `it` of task t. All 256 threads participate.
-__device__ __forceinline__ void exec_item(const DevTask& t, int it, const DevMat* mats, char* lds) {
+// Item variants a kernel is compiled with (bit mask). A kernel holding every
+// variant allocates registers for the largest one; the executors are therefore
+// instantiated per precision and the host picks the smallest set that covers a graph.
+enum ItemVariant : int {
+    V_F64S = 1,      // FP64 tiles: POTRF, TRSM, GEMM/SYRK on FP64 MFMA
+    V_F32S = 2,      // FP32 tiles: POTRF (FP64 arithmetic), TRSM on FP32 MFMA
+    V_G32 = 4,       // GEMM/SYRK FP32
+    V_GBF16 = 8,     // GEMM/SYRK BF16
+    V_GBF16X3 = 16,  // GEMM/SYRK BF16x3
+    V_ALL = 31
+};
+// The compiled sets, smallest first.
+constexpr int N_VSETS = 5;
+constexpr int VSETS[N_VSETS] = {V_F64S, V_F32S | V_G32, V_F32S | V_GBF16, V_F32S | V_GBF16X3, V_ALL};
+
+__host__ __device__ inline int item_variant(int kind, int prec, int storage) {
+    if (kind == K_GEMM || kind == K_SYRK)
+        return prec == P_F64 ? V_F64S : prec == P_F32 ? V_G32 : prec == P_BF16 ? V_GBF16 : V_GBF16X3;
+    return storage == P_F64 ? V_F64S : V_F32S;
+}
+inline int vset_index(int needed) {
+    for (int q = 0; q < N_VSETS; q++)
+        if ((needed & ~VSETS[q]) == 0) return q;
+    return N_VSETS - 1;
+}
+
+// Execute work item `it` of task t. All 256 threads participate. Returns false
+// (doing nothing) for an item variant outside VM.
+template <int VM>
+__device__ __forceinline__ bool exec_item(const DevTask& t, int it, const DevMat* mats, char* lds) {
     const DevMat& M = mats[t.mat];
     const int nt = M.nt;
     auto tile = [&](int i, int j) { return M.tiles[i + (long)j * nt]; };
+    const bool f64s = M.storage == P_F64;
     switch (t.kind) {
         case K_POTRF: {
             const int j = t.j, n = M.bsz[j];
-            if (M.storage == P_F64) item_potrf<double>((double*)tile(j, j), n, n, M.dinv[j], M.info, M.boff[j], lds);
-            else item_potrf<float>((float*)tile(j, j), n, n, M.dinv[j], M.info, M.boff[j], lds);
-            break;
+            if constexpr ((VM & V_F64S) != 0)
+                if (f64s) {
+                    item_potrf<double>((double*)tile(j, j), n, n, M.dinv[j], M.info, M.boff[j], lds);
+                    return true;
+                }
+            if constexpr ((VM & V_F32S) != 0)
+                if (!f64s) {
+                    item_potrf<float>((float*)tile(j, j), n, n, M.dinv[j], M.info, M.boff[j], lds);
+                    return true;
+                }
+            return false;
         }
         case K_TRSM: {
             const int i = t.i, j = t.j, m = M.bsz[i], n = M.bsz[j];
-            if (M.storage == P_F64)
-                item_trsm<double, double>((const double*)tile(j, j), n, M.dinv[j], (double*)tile(i, j), m, m, n,
-                                          it * TRSM_BM, lds);
-            else
-                item_trsm<float, float>((const float*)tile(j, j), n, M.dinv[j], (float*)tile(i, j), m, m, n,
-                                        it * TRSM_BM, lds);
-            break;
+            if constexpr ((VM & V_F64S) != 0)
+                if (f64s) {
+                    item_trsm<double, double>((const double*)tile(j, j), n, M.dinv[j], (double*)tile(i, j), m, m, n,
+                                              it * TRSM_BM, lds);
+                    return true;
+                }
+            if constexpr ((VM & V_F32S) != 0)
+                if (!f64s) {
+                    item_trsm<float, float>((const float*)tile(j, j), n, M.dinv[j], (float*)tile(i, j), m, m, n,
+                                            it * TRSM_BM, lds);
+                    return true;
+                }
+            return false;
         }
         case K_SYRK:
         case K_GEMM: {
@@ -69,26 +112,35 @@ __device__ __forceinline__ void exec_item(const DevTask& t, int it, const DevMat
             const bool syrk = (t.kind == K_SYRK);
             int rb, cb;
             decode_gemm_item(it, (m + GEMM_BM - 1) / GEMM_BM, syrk, rb, cb);
-            if (t.prec == P_F64)
-                item_gemm<double, P_F64>((const double*)tile(i, k), m, (const double*)tile(j, k), n,
-                                         (double*)tile(i, j), m, m, n, K, rb * GEMM_BM, cb * GEMM_BM, syrk, -1.0,
-                                         1.0, lds);
-            else if (t.prec == P_F32)
-                item_gemm<float, P_F32>((const float*)tile(i, k), m, (const float*)tile(j, k), n,
-                                        (float*)tile(i, j), m, m, n, K, rb * GEMM_BM, cb * GEMM_BM, syrk, -1.0, 1.0,
-                                        lds);
-            else if (t.prec == P_BF16)
-                item_gemm<float, P_BF16>((const float*)tile(i, k), m, (const float*)tile(j, k), n,
-                                         (float*)tile(i, j), m, m, n, K, rb * GEMM_BM, cb * GEMM_BM, syrk, -1.0,
-                                         1.0, lds);
-            else
-                item_gemm<float, P_BF16X3>((const float*)tile(i, k), m, (const float*)tile(j, k), n,
-                                           (float*)tile(i, j), m, m, n, K, rb * GEMM_BM, cb * GEMM_BM, syrk, -1.0,
-                                           1.0, lds);
-            break;
+            const int r0 = rb * GEMM_BM, c0 = cb * GEMM_BM;
+            if constexpr ((VM & V_F64S) != 0)
+                if (t.prec == P_F64) {
+                    item_gemm<double, P_F64>((const double*)tile(i, k), m, (const double*)tile(j, k), n,
+                                             (double*)tile(i, j), m, m, n, K, r0, c0, syrk, -1.0, 1.0, lds);
+                    return true;
+                }
+            if constexpr ((VM & V_G32) != 0)
+                if (t.prec == P_F32) {
+                    item_gemm<float, P_F32>((const float*)tile(i, k), m, (const float*)tile(j, k), n,
+                                            (float*)tile(i, j), m, m, n, K, r0, c0, syrk, -1.0, 1.0, lds);
+                    return true;
+                }
+            if constexpr ((VM & V_GBF16) != 0)
+                if (t.prec == P_BF16) {
+                    item_gemm<float, P_BF16>((const float*)tile(i, k), m, (const float*)tile(j, k), n,
+                                             (float*)tile(i, j), m, m, n, K, r0, c0, syrk, -1.0, 1.0, lds);
+                    return true;
+                }
+            if constexpr ((VM & V_GBF16X3) != 0)
+                if (t.prec == P_BF16X3) {
+                    item_gemm<float, P_BF16X3>((const float*)tile(i, k), m, (const float*)tile(j, k), n,
+                                               (float*)tile(i, j), m, m, n, K, r0, c0, syrk, -1.0, 1.0, lds);
+                    return true;
+                }
+            return false;
         }
         default:
-            break;
+            return false;
     }
 }
 

@@ -1,4 +1,6 @@
 // HIP kernels of the tiled Cholesky library (gfx950).
+#include <algorithm>
+
 #include "kernels.hpp"
 #include "executor.hpp"
 
@@ -8,8 +10,9 @@ namespace ttad {
 // Host-sequenced executor: one block per work item of a batch of tasks.
 // prefix[t] = first global item of task t (exclusive scan of n_items).
 // ---------------------------------------------------------------------------
+template <int VM>
 __global__ __launch_bounds__(NTHREADS) void k_items(const DevMat* mats, const DevTask* tasks, const int32_t* prefix,
-                                                    int n_tasks) {
+                                                    int n_tasks, int32_t* err) {
     extern __shared__ __align__(16) char lds[];
     const int g = blockIdx.x;
     int lo = 0, hi = n_tasks - 1;
@@ -19,7 +22,7 @@ __global__ __launch_bounds__(NTHREADS) void k_items(const DevMat* mats, const De
         else hi = mid - 1;
     }
     const DevTask t = tasks[lo];
-    exec_item(t, g - prefix[lo], mats, lds);
+    if (!exec_item<VM>(t, g - prefix[lo], mats, lds) && threadIdx.x == 0) atomicCAS(err, 0, 3);
 }
 
 // ---------------------------------------------------------------------------
@@ -49,6 +52,7 @@ __device__ __forceinline__ void push_task(const DevGraph& g, int s) {
                            __HIP_MEMORY_SCOPE_AGENT);
 }
 
+template <int VM>
 __global__ __launch_bounds__(NTHREADS) void k_persistent(DevGraph g) {
     extern __shared__ __align__(16) char lds[];
     __shared__ int64_t s_item;
@@ -120,10 +124,11 @@ __global__ __launch_bounds__(NTHREADS) void k_persistent(DevGraph g) {
         const DevTask t = g.tasks[task];
         uint64_t t0 = 0;
         if (g.trace && tid == 0) t0 = wall_clock64();
-        exec_item(t, item, g.mats, lds);
+        const bool ran = exec_item<VM>(t, item, g.mats, lds);
         __syncthreads();
         if (tid == 0) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (!ran) __hip_atomic_store(g.abort_flag, 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (g.trace) {
                 const int slot = __hip_atomic_fetch_add(g.trace_cursor, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (slot < g.total_items) {
@@ -205,15 +210,16 @@ __global__ __launch_bounds__(NTHREADS) void k_raw_trsm(const TS* L, long ldl, co
 template <typename TS>
 __global__ __launch_bounds__(NTHREADS) void k_trtri_diag(const TS* L, long ldl, int n, double* dinv) {
     __shared__ double D[DB * DP];
+    __shared__ double Xs[DB * DP];
     const int pb = blockIdx.x, p0 = pb * DB, pw = min(DB, n - p0);
     for (int e = threadIdx.x; e < DB * DB; e += NTHREADS) {
         const int i = e % DB, j = e / DB;
-        D[i + j * DP] = (i < pw && j < pw) ? (double)L[(p0 + i) + (long)(p0 + j) * ldl] : (i == j ? 1.0 : 0.0);
+        D[i + j * DP] = (i < pw && j < pw && i >= j) ? (double)L[(p0 + i) + (long)(p0 + j) * ldl] : (i == j ? 1.0 : 0.0);
     }
     __syncthreads();
-    lds_trtri64(D, pw);
+    lds_trtri64(D, Xs, pw);
     double* X = dinv + (long)pb * DB * DB;
-    for (int e = threadIdx.x; e < DB * DB; e += NTHREADS) X[e] = D[(e % DB) + (e / DB) * DP];
+    for (int e = threadIdx.x; e < DB * DB; e += NTHREADS) X[e] = Xs[(e % DB) + (e / DB) * DP];
 }
 
 // ---------------------------------------------------------------------------
@@ -280,8 +286,16 @@ hipError_t kernels_init() {
         e = set_lds((const void*)(fn), LDS_ITEM_MAX);            \
         if (e != hipSuccess) return e;                           \
     } while (0)
-    TTAD_SETLDS(k_items);
-    TTAD_SETLDS(k_persistent);
+    TTAD_SETLDS(k_items<VSETS[0]>);
+    TTAD_SETLDS(k_items<VSETS[1]>);
+    TTAD_SETLDS(k_items<VSETS[2]>);
+    TTAD_SETLDS(k_items<VSETS[3]>);
+    TTAD_SETLDS(k_items<VSETS[4]>);
+    TTAD_SETLDS(k_persistent<VSETS[0]>);
+    TTAD_SETLDS(k_persistent<VSETS[1]>);
+    TTAD_SETLDS(k_persistent<VSETS[2]>);
+    TTAD_SETLDS(k_persistent<VSETS[3]>);
+    TTAD_SETLDS(k_persistent<VSETS[4]>);
     TTAD_SETLDS((k_raw_gemm<double, P_F64>));
     TTAD_SETLDS((k_raw_gemm<float, P_F32>));
     TTAD_SETLDS((k_raw_gemm<float, P_BF16>));
@@ -294,18 +308,39 @@ hipError_t kernels_init() {
     return e;
 }
 
-int persistent_blocks_per_cu() {
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_persistent, NTHREADS, LDS_ITEM_MAX) !=
-        hipSuccess)
-        return 1;
-    return nb > 0 ? nb : 1;
+static const void* persistent_fn(int vset) {
+    switch (vset) {
+        case 0: return (const void*)k_persistent<VSETS[0]>;
+        case 1: return (const void*)k_persistent<VSETS[1]>;
+        case 2: return (const void*)k_persistent<VSETS[2]>;
+        case 3: return (const void*)k_persistent<VSETS[3]>;
+        default: return (const void*)k_persistent<VSETS[4]>;
+    }
 }
 
-void launch_items(const DevMat* mats, const DevTask* tasks, const int32_t* prefix, int n_tasks, int n_items,
-                  hipStream_t s) {
+int persistent_blocks_per_cu() {
+    int best = 1 << 30;  // the smallest over the variants: one worker count fits every graph
+    for (int q = 0; q < N_VSETS; q++) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, persistent_fn(q), NTHREADS, LDS_ITEM_MAX) !=
+            hipSuccess)
+            return 1;
+        best = std::min(best, nb);
+    }
+    return best > 0 ? best : 1;
+}
+
+void launch_items(int vset, const DevMat* mats, const DevTask* tasks, const int32_t* prefix, int n_tasks,
+                  int n_items, int32_t* err, hipStream_t s) {
     if (n_items <= 0) return;
-    hipLaunchKernelGGL(k_items, dim3(n_items), dim3(NTHREADS), LDS_ITEM_MAX, s, mats, tasks, prefix, n_tasks);
+    const dim3 grid(n_items), block(NTHREADS);
+    switch (vset) {
+        case 0: hipLaunchKernelGGL(k_items<VSETS[0]>, grid, block, LDS_ITEM_MAX, s, mats, tasks, prefix, n_tasks, err); break;
+        case 1: hipLaunchKernelGGL(k_items<VSETS[1]>, grid, block, LDS_ITEM_MAX, s, mats, tasks, prefix, n_tasks, err); break;
+        case 2: hipLaunchKernelGGL(k_items<VSETS[2]>, grid, block, LDS_ITEM_MAX, s, mats, tasks, prefix, n_tasks, err); break;
+        case 3: hipLaunchKernelGGL(k_items<VSETS[3]>, grid, block, LDS_ITEM_MAX, s, mats, tasks, prefix, n_tasks, err); break;
+        default: hipLaunchKernelGGL(k_items<VSETS[4]>, grid, block, LDS_ITEM_MAX, s, mats, tasks, prefix, n_tasks, err); break;
+    }
 }
 
 void launch_graph_reset(const DevGraph& g, int n_slots, hipStream_t s) {
@@ -313,8 +348,15 @@ void launch_graph_reset(const DevGraph& g, int n_slots, hipStream_t s) {
     hipLaunchKernelGGL(k_graph_reset_b, dim3(64), dim3(256), 0, s, g);
 }
 
-void launch_persistent(const DevGraph& g, int n_workers, hipStream_t s) {
-    hipLaunchKernelGGL(k_persistent, dim3(n_workers), dim3(NTHREADS), LDS_ITEM_MAX, s, g);
+void launch_persistent(int vset, const DevGraph& g, int n_workers, hipStream_t s) {
+    const dim3 grid(n_workers), block(NTHREADS);
+    switch (vset) {
+        case 0: hipLaunchKernelGGL(k_persistent<VSETS[0]>, grid, block, LDS_ITEM_MAX, s, g); break;
+        case 1: hipLaunchKernelGGL(k_persistent<VSETS[1]>, grid, block, LDS_ITEM_MAX, s, g); break;
+        case 2: hipLaunchKernelGGL(k_persistent<VSETS[2]>, grid, block, LDS_ITEM_MAX, s, g); break;
+        case 3: hipLaunchKernelGGL(k_persistent<VSETS[3]>, grid, block, LDS_ITEM_MAX, s, g); break;
+        default: hipLaunchKernelGGL(k_persistent<VSETS[4]>, grid, block, LDS_ITEM_MAX, s, g); break;
+    }
 }
 
 void launch_raw_gemm(int prec, const void* A, long lda, const void* B, long ldb, void* C, long ldc, int m, int n,

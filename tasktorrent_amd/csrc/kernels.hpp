@@ -9,10 +9,11 @@ struct DevGraph;
 
 hipError_t kernels_init();
 int persistent_blocks_per_cu();
-void launch_items(const DevMat* mats, const DevTask* tasks, const int32_t* prefix, int n_tasks, int n_items,
-                  hipStream_t s);
+// vset: index into the compiled item-variant sets (vset_index of the variants a graph needs).
+void launch_items(int vset, const DevMat* mats, const DevTask* tasks, const int32_t* prefix, int n_tasks,
+                  int n_items, int32_t* err, hipStream_t s);
 void launch_graph_reset(const DevGraph& g, int n_slots, hipStream_t s);
-void launch_persistent(const DevGraph& g, int n_workers, hipStream_t s);
+void launch_persistent(int vset, const DevGraph& g, int n_workers, hipStream_t s);
 void launch_raw_gemm(int prec, const void* A, long lda, const void* B, long ldb, void* C, long ldc, int m, int n,
                      int K, int syrk, double alpha, double beta, hipStream_t s);
 void launch_raw_potrf(int prec, void* A, long lda, int n, double* dinv, int* info, hipStream_t s);

@@ -102,6 +102,7 @@ struct ttad_graph {
         d_q_base, d_q_items, d_done, d_abort, d_mats, d_trace, d_trace_cursor;
     int n_init = 0;
     int workers = 0;
+    int vset = 4;  // executor instantiation (index into the item-variant sets) covering the graph
     bool traced = false;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     hipStream_t stream = nullptr;

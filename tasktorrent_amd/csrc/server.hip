@@ -38,8 +38,9 @@ __device__ __forceinline__ int64_t srv_try_pop(const DevServer& s) {
         for (;;) {
             const unsigned long long e = __hip_atomic_load(slot, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
             if ((e >> 40) == lap) return (int64_t)(e & 0xFFFFFFFFFFull);
-            if ((int64_t)wall_clock64() - t0 > s.spin_limit) {
-                __hip_atomic_store(s.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((int64_t)wall_clock64() - t0 > s.spin_limit ||
+                __hip_atomic_load(s.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0) {
+                __hip_atomic_store(s.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                 return -2;
             }
             __builtin_amdgcn_s_sleep(1);
@@ -572,11 +573,13 @@ int ttad_server_destroy(ttad_server* S) {
     if (!S) return TTAD_OK;
     int rc = TTAD_OK;
     if (S->running) {
-        // graceful: finish the requests in flight; forced after 30 s (workers exit when idle)
+        // graceful: finish the requests in flight; forced after 30 s (err ends every device
+        // wait, workers exit at their next poll)
         __atomic_store_n(S->shutdown, 1, __ATOMIC_RELEASE);
         auto t0 = std::chrono::steady_clock::now();
         while (hipStreamQuery(S->s_run) == hipErrorNotReady) {
             if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > 30.0) {
+                if (__atomic_load_n(S->err, __ATOMIC_ACQUIRE) == 0) __atomic_store_n(S->err, 4, __ATOMIC_RELEASE);
                 __atomic_store_n(S->shutdown, 2, __ATOMIC_RELEASE);
                 rc = set_error(TTAD_ERR_TIMEOUT, "server_destroy: requests still in flight after 30 s; forced stop");
                 break;
@@ -605,18 +608,18 @@ struct MailBuilder {
     long long used = 0;
     std::vector<SrvCopy> copies;
     bool ok = true;
+    // Exact sizes: the device writes `bytes` and nothing after them (the next words of a
+    // task or edge range may belong to a request that is running).
     void add(void* dst_dev, const void* src, size_t bytes) {
-        const size_t b8 = (bytes + 7) / 8 * 8;
         used = (used + 255) / 256 * 256;
-        if (used + (long long)b8 > cap) {
+        if (bytes % 4 || (unsigned long long)dst_dev % 4 || used + (long long)bytes > cap) {
             ok = false;
             return;
         }
         char* h = S->mbox + off + used;
         memcpy(h, src, bytes);
-        if (b8 > bytes) memset(h + bytes, 0, b8 - bytes);
-        copies.push_back(SrvCopy{(unsigned long long)dst_dev, S->dev_of(h), (unsigned long long)b8});
-        used += (long long)b8;
+        copies.push_back(SrvCopy{(unsigned long long)dst_dev, S->dev_of(h), (unsigned long long)bytes});
+        used += (long long)bytes;
     }
     unsigned long long finish() {  // the copy list itself, at the end of the range
         used = (used + 255) / 256 * 256;

@@ -51,9 +51,9 @@ struct SrvReq {
 // Admission mail of one request slot (pinned host memory, written by the host, read by
 // the workgroup that admits the request): a list of copies host -> HBM, then the seed.
 struct SrvCopy {
-    unsigned long long dst;     // device address
+    unsigned long long dst;     // device address (4-byte aligned)
     unsigned long long src;     // device address of pinned host memory
-    unsigned long long bytes;   // multiple of 8
+    unsigned long long bytes;   // exact payload size, a multiple of 4: nothing past it is written
 };
 struct SrvMail {
     int32_t state;              // 0 free, 1 posted by the host, 2 copied by the device
@@ -97,7 +97,7 @@ struct DevServer {
     int32_t db_cap;
     int32_t* db_taken;          // device: doorbell entries consumed
     int32_t* active;            // device: requests in flight
-    int32_t* err;               // host pinned: watchdog codes
+    int32_t* err;               // host pinned: watchdog codes; nonzero also ends every device wait early
     int64_t spin_limit;
     int64_t idle_limit;
 };
@@ -145,10 +145,10 @@ __device__ __forceinline__ void srv_admit(const DevServer& s, int slot) {
     // every wave reads the host's payloads after thread 0's acquire of the doorbell
     __atomic_thread_fence(__ATOMIC_ACQUIRE);  // system scope
     const SrvCopy* cl = (const SrvCopy*)s_copies;
-    for (int c = 0; c < s_nc; c++) {
-        const unsigned long long* src = (const unsigned long long*)cl[c].src;
-        unsigned long long* dst = (unsigned long long*)cl[c].dst;
-        const long words = (long)(cl[c].bytes >> 3);
+    for (int c = 0; c < s_nc; c++) {  // 4-byte words: task and edge ranges start at any entry
+        const uint32_t* src = (const uint32_t*)cl[c].src;
+        uint32_t* dst = (uint32_t*)cl[c].dst;
+        const long words = (long)(cl[c].bytes >> 2);
         for (long w = threadIdx.x; w < words; w += NTHREADS) dst[w] = src[w];
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // each wave publishes its own copies
@@ -312,7 +312,7 @@ __device__ __forceinline__ void srv_exec(const DevServer& s, const SrvTask& t, i
             d.i = t.i;
             d.j = t.j;
             d.k = t.k;
-            exec_item(d, it, s.mats, lds);
+            (void)exec_item<V_ALL>(d, it, s.mats, lds);  // request graphs only hold supported kinds
             break;
         }
     }

@@ -35,7 +35,10 @@ static SweepTab tab_of(const ttad_matrix* M) {
 }
 
 static int ensure(DevBuf& b, size_t bytes) {
-    if (b.bytes < bytes) TTAD_HIP(b.alloc(bytes));
+    if (b.bytes < bytes) {  // new buffers start zeroed (the sweep error flag relies on it)
+        TTAD_HIP(b.alloc(bytes));
+        TTAD_HIP(hipMemset(b.p, 0, b.bytes));
+    }
     return TTAD_OK;
 }
 
@@ -170,7 +173,11 @@ static int check_err_flag(ttad_context* ctx, hipStream_t s) {
     int32_t e = 0;
     TTAD_HIP(hipMemcpyAsync(&e, ctx->sw_err.p, 4, hipMemcpyDeviceToHost, s));
     TTAD_HIP(hipStreamSynchronize(s));
-    if (e) return set_error(TTAD_ERR_TIMEOUT, "solve: a sweep block waited past the device watchdog");
+    if (e) {  // cleared for the next call: a set flag also makes every device wait give up at once
+        TTAD_HIP(hipMemsetAsync(ctx->sw_err.p, 0, 4, s));
+        TTAD_HIP(hipStreamSynchronize(s));
+        return set_error(TTAD_ERR_TIMEOUT, "solve: a sweep block waited past the device watchdog");
+    }
     return TTAD_OK;
 }
 

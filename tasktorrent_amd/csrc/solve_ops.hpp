@@ -99,11 +99,15 @@ __device__ __forceinline__ void block_mma(typename Mfma<TC>::acc_t (&acc)[NR / 1
     }
 }
 
+// Waits for *f != 0. Gives up after `limit` ticks (sets *err = 1) or as soon as *err is
+// nonzero (an earlier wait timed out, or the owner asked the kernel to stop), so that a
+// failed run drains in one time limit rather than one per flag.
 __device__ __forceinline__ void spin_flag(const int32_t* f, int32_t* err, int64_t limit) {
     const int64_t t0 = (int64_t)wall_clock64();
     while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0) return;
         if ((int64_t)wall_clock64() - t0 > limit) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             return;
         }
         __builtin_amdgcn_s_sleep(1);

@@ -42,22 +42,17 @@ template <> struct Mfma<float> {
     }
 };
 
-__device__ __forceinline__ unsigned short f2bf_bits(float x) {
-    // round to nearest even; NaN kept quiet
-    unsigned u = __float_as_uint(x);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
-
 template <typename T> __device__ __forceinline__ T zero_val() { return T(0); }
 
 // ---------------------------------------------------------------------------
 // acc (BM x BN) += A(BM x K) * B(BN x K)^T with A, B column-major (lda, ldb),
 // rows >= mv of A and rows >= nv of B read as zero. TA/TB are storage types,
-// TC the MFMA type; conversion happens on the global -> LDS path.
+// TC the MFMA type; conversion happens on the register -> LDS path.
 // 2x2 waves, each wave owns a (BM/2) x (BN/2) block of 16x16 MFMA tiles.
-// K is staged through LDS in chunks of BK with a one-chunk register prefetch.
+// K is staged through two LDS buffers in chunks of BK: while the MFMAs consume
+// chunk c, chunk c+1 goes from registers to the other buffer and the global
+// loads of chunk c+2 are in flight (one barrier per chunk). A chunk's MFMA time
+// (BK=32 FP64: 8 k-steps x 16 MFMAs per wave) is what hides the load latency.
 // ---------------------------------------------------------------------------
 template <typename TA, typename TB, typename TC, int BM, int BN, int BK>
 struct GemmNT {
@@ -67,9 +62,11 @@ struct GemmNT {
     static constexpr int WN = BN / 32;
     static constexpr int EA = BM * BK / NTHREADS;
     static constexpr int EB = BN * BK / NTHREADS;
-    static constexpr int LDS_BYTES = BK * (PA + PB) * (int)sizeof(TC);
+    static constexpr int STAGE = BK * (PA + PB);  // TC elements of one LDS buffer
+    static constexpr int LDS_BYTES = 2 * STAGE * (int)sizeof(TC);
     typedef typename Mfma<TC>::acc_t acc_t;
     static_assert(EA * NTHREADS == BM * BK && EB * NTHREADS == BN * BK, "tile/thread mismatch");
+    static_assert(BK % 4 == 0, "BK is a multiple of the MFMA k-step");
 
     static __device__ __forceinline__ void zero(acc_t (&acc)[WM][WN]) {
 #pragma unroll
@@ -87,25 +84,55 @@ struct GemmNT {
         return wc * (WN * 16) + b * 16 + (lane & 15);
     }
 
-    // Each thread stages a fixed row (tid % BM) and every (NTHREADS/BM)-th k of a chunk.
+    // Each thread stages a fixed row (tid % BM) and every (NTHREADS/BM)-th k of a chunk:
+    // one wave instruction reads 64 consecutive rows of one column (coalesced).
     static constexpr int KS_A = NTHREADS / BM;
     static constexpr int KS_B = NTHREADS / BN;
     static_assert(NTHREADS % BM == 0 && NTHREADS % BN == 0, "BM, BN must divide the workgroup");
 
     template <typename T, int E, int KS>
-    static __device__ __forceinline__ void load_chunk(TC (&reg)[E], const T* __restrict__ base, long ld, bool row_ok,
+    static __device__ __forceinline__ void load_chunk(T (&reg)[E], const T* __restrict__ base, long ld, bool row_ok,
                                                       int kfirst, int K) {
 #pragma unroll
         for (int s = 0; s < E; s++) {
             const int kk = kfirst + s * KS;
-            reg[s] = (row_ok && kk < K) ? (TC)base[(long)kk * ld] : zero_val<TC>();
+            reg[s] = (row_ok && kk < K) ? base[(long)kk * ld] : T(0);
         }
     }
 
+    static __device__ __forceinline__ void store_chunk(TC* __restrict__ S, const TA (&ra)[EA], const TB (&rb)[EB],
+                                                       int ra_row, int ra_k, int rb_row, int rb_k) {
+        TC* As = S;
+        TC* Bs = S + BK * PA;
+#pragma unroll
+        for (int s = 0; s < EA; s++) As[(ra_k + s * KS_A) * PA + ra_row] = (TC)ra[s];
+#pragma unroll
+        for (int s = 0; s < EB; s++) Bs[(rb_k + s * KS_B) * PB + rb_row] = (TC)rb[s];
+    }
+
+    static __device__ __forceinline__ void compute(acc_t (&acc)[WM][WN], const TC* __restrict__ S, int lane, int wr,
+                                                   int wc) {
+        const TC* As = S + wr * (WM * 16) + (lane & 15);
+        const TC* Bs = S + BK * PA + wc * (WN * 16) + (lane & 15);
+#pragma unroll 4
+        for (int ks = 0; ks < BK / 4; ks++) {
+            const int kr = ks * 4 + (lane >> 4);
+            TC av[WM], bv[WN];
+#pragma unroll
+            for (int a = 0; a < WM; a++) av[a] = As[kr * PA + a * 16];
+#pragma unroll
+            for (int b = 0; b < WN; b++) bv[b] = Bs[kr * PB + b * 16];
+#pragma unroll
+            for (int a = 0; a < WM; a++)
+#pragma unroll
+                for (int b = 0; b < WN; b++) acc[a][b] = Mfma<TC>::mma(av[a], bv[b], acc[a][b]);
+        }
+    }
+
+    // Starts and ends with a workgroup barrier: the caller may use `lds` before and after.
     static __device__ __forceinline__ void run(acc_t (&acc)[WM][WN], const TA* __restrict__ A, long lda, int mv,
                                const TB* __restrict__ B, long ldb, int nv, int K, char* lds) {
-        TC* As = reinterpret_cast<TC*>(lds);
-        TC* Bs = As + BK * PA;
+        TC* S0 = reinterpret_cast<TC*>(lds);
         const int tid = threadIdx.x;
         const int lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
         const int ra_row = tid % BM, ra_k = tid / BM;
@@ -113,61 +140,65 @@ struct GemmNT {
         const bool a_ok = ra_row < mv, b_ok = rb_row < nv;
         const TA* Ab = A + ra_row;
         const TB* Bb = B + rb_row;
-        TC ra[EA], rb[EB];
         const int nchunks = (K + BK - 1) / BK;
-        if (nchunks > 0) {
-            load_chunk<TA, EA, KS_A>(ra, Ab, lda, a_ok, ra_k, K);
-            load_chunk<TB, EB, KS_B>(rb, Bb, ldb, b_ok, rb_k, K);
-        }
-        for (int c = 0; c < nchunks; c++) {
-            __syncthreads();
-#pragma unroll
-            for (int s = 0; s < EA; s++) As[(ra_k + s * KS_A) * PA + ra_row] = ra[s];
-#pragma unroll
-            for (int s = 0; s < EB; s++) Bs[(rb_k + s * KS_B) * PB + rb_row] = rb[s];
-            __syncthreads();
-            if (c + 1 < nchunks) {
-                const int k0 = (c + 1) * BK;
-                load_chunk<TA, EA, KS_A>(ra, Ab, lda, a_ok, k0 + ra_k, K);
-                load_chunk<TB, EB, KS_B>(rb, Bb, ldb, b_ok, k0 + rb_k, K);
-            }
-#pragma unroll 2
-            for (int ks = 0; ks < BK / 4; ks++) {
-                const int kr = ks * 4 + (lane >> 4);
-                TC av[WM], bv[WN];
-#pragma unroll
-                for (int a = 0; a < WM; a++) av[a] = As[kr * PA + wr * (WM * 16) + a * 16 + (lane & 15)];
-#pragma unroll
-                for (int b = 0; b < WN; b++) bv[b] = Bs[kr * PB + wc * (WN * 16) + b * 16 + (lane & 15)];
-#pragma unroll
-                for (int a = 0; a < WM; a++)
-#pragma unroll
-                    for (int b = 0; b < WN; b++) acc[a][b] = Mfma<TC>::mma(av[a], bv[b], acc[a][b]);
-            }
+        __syncthreads();
+        if (nchunks == 0) return;
+        TA ra[EA];
+        TB rb[EB];
+        load_chunk<TA, EA, KS_A>(ra, Ab, lda, a_ok, ra_k, K);
+        load_chunk<TB, EB, KS_B>(rb, Bb, ldb, b_ok, rb_k, K);
+        store_chunk(S0, ra, rb, ra_row, ra_k, rb_row, rb_k);
+        if (nchunks > 1) {
+            load_chunk<TA, EA, KS_A>(ra, Ab, lda, a_ok, BK + ra_k, K);
+            load_chunk<TB, EB, KS_B>(rb, Bb, ldb, b_ok, BK + rb_k, K);
         }
         __syncthreads();
+        for (int c = 0; c < nchunks; c++) {
+            compute(acc, S0 + (c & 1) * STAGE, lane, wr, wc);
+            if (c + 1 < nchunks) {
+                // the other buffer was last read in chunk c-1, before the previous barrier
+                store_chunk(S0 + ((c + 1) & 1) * STAGE, ra, rb, ra_row, ra_k, rb_row, rb_k);
+                if (c + 2 < nchunks) {
+                    const int k0 = (c + 2) * BK;
+                    load_chunk<TA, EA, KS_A>(ra, Ab, lda, a_ok, k0 + ra_k, K);
+                    load_chunk<TB, EB, KS_B>(rb, Bb, ldb, b_ok, k0 + rb_k, K);
+                }
+            }
+            __syncthreads();
+        }
     }
 };
 
 // ---------------------------------------------------------------------------
 // BF16 inputs, FP32 accumulation: acc (BM x BN) += bf16(A) * bf16(B)^T.
 // LDS holds bf16 in [row][k] order so that one lane reads its 8 consecutive k
-// values with a single 16-byte load.
+// values with a single 16-byte load. Raw FP32 values are prefetched one chunk
+// ahead in registers and converted (v_cvt_pk_bf16_f32, round to nearest even)
+// on their way into the LDS buffer not being read; two buffers, one barrier per chunk.
 // ---------------------------------------------------------------------------
 // SPLIT = true: each FP32 operand x is split into hi = bf16(x) and lo = bf16(x - hi)
 // and the product accumulates hi*hi + hi*lo + lo*hi (three MFMAs), which keeps about
 // 16 significant bits of every operand (Henry, Tang, Heinecke, "Leveraging the
 // bfloat16 artificial intelligence datatype for higher-precision computations", 2019).
+typedef __bf16 bf2_t __attribute__((ext_vector_type(2)));
+typedef float f2_t __attribute__((ext_vector_type(2)));
+typedef unsigned u4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned pack_bf16x2(float a, float b) {
+    return __builtin_bit_cast(unsigned, __builtin_convertvector((f2_t){a, b}, bf2_t));
+}
+
 template <typename TA, int BM, int BN, bool SPLIT = false>
 struct GemmNTbf16 {
-    static constexpr int BK = 32;
+    static constexpr int BK = 64;
     static constexpr int PK = BK + 8;  // bf16 elements per LDS row (16-byte aligned, skewed)
     static constexpr int WM = BM / 32;
     static constexpr int WN = BN / 32;
     static constexpr int NP = SPLIT ? 2 : 1;  // hi (and lo) planes
-    static constexpr int LDS_BYTES = NP * (BM + BN) * PK * 2;
+    static constexpr int STAGE = NP * (BM + BN) * PK;  // bf16 elements of one LDS buffer
+    static constexpr int LDS_BYTES = 2 * STAGE * 2;
     typedef f4_t acc_t;
-    // each thread converts 8 consecutive k of one row per step
+    // each thread owns groups of 8 consecutive k of one row
     static constexpr int GA = BM * (BK / 8) / NTHREADS;
     static constexpr int GB = BN * (BK / 8) / NTHREADS;
     static_assert(GA * NTHREADS == BM * (BK / 8) && GB * NTHREADS == BN * (BK / 8), "tile/thread mismatch");
@@ -188,61 +219,46 @@ struct GemmNTbf16 {
     }
 
     template <int G, int GS>
-    static __device__ __forceinline__ void load8(s8_t (&hi)[G], s8_t (&lo)[G], const TA* __restrict__ base, long ld,
-                                                 bool row_ok, int kfirst, int K) {
+    static __device__ __forceinline__ void load_raw(float (&r)[G][8], const TA* __restrict__ base, long ld,
+                                                    bool row_ok, int kfirst, int K) {
 #pragma unroll
         for (int s = 0; s < G; s++) {
             const int k = kfirst + s * GS * 8;
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const float x = (row_ok && k + q < K) ? (float)base[(long)(k + q) * ld] : 0.0f;
-                const unsigned short h = f2bf_bits(x);
-                hi[s][q] = (short)h;
-                if constexpr (SPLIT) lo[s][q] = (short)f2bf_bits(x - __uint_as_float((unsigned)h << 16));
-            }
+            for (int q = 0; q < 8; q++) r[s][q] = (row_ok && k + q < K) ? (float)base[(long)(k + q) * ld] : 0.0f;
         }
     }
 
-    static __device__ __forceinline__ void run(acc_t (&acc)[WM][WN], const TA* __restrict__ A, long lda, int mv,
-                               const TA* __restrict__ B, long ldb, int nv, int K, char* lds) {
-        unsigned short* As = reinterpret_cast<unsigned short*>(lds);  // hi plane of A
-        unsigned short* Bs = As + BM * PK;                            // hi plane of B
-        unsigned short* Al = Bs + BN * PK;                            // lo planes (SPLIT)
-        unsigned short* Bl = Al + BM * PK;
-        const int tid = threadIdx.x;
-        const int lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
-        // thread -> fixed row (tid % BM) and k-groups of 8 (tid / BM + s * NTHREADS / BM)
-        const int a_row = tid % BM, a_g = tid / BM, b_row = tid % BN, b_g = tid / BN;
-        const bool a_ok = a_row < mv, b_ok = b_row < nv;
-        const TA* Ab = A + a_row;
-        const TA* Bb = B + b_row;
-        s8_t ra[GA], rb[GB], la[GA], lb[GB];
-        const int nchunks = (K + BK - 1) / BK;
-        if (nchunks > 0) {
-            load8<GA, NTHREADS / BM>(ra, la, Ab, lda, a_ok, a_g * 8, K);
-            load8<GB, NTHREADS / BN>(rb, lb, Bb, ldb, b_ok, b_g * 8, K);
+    template <int G, int GS>
+    static __device__ __forceinline__ void store_planes(unsigned short* __restrict__ hi_plane,
+                                                        unsigned short* __restrict__ lo_plane, const float (&r)[G][8],
+                                                        int row, int g0) {
+#pragma unroll
+        for (int s = 0; s < G; s++) {
+            u4_t h, l;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const float x0 = r[s][2 * q], x1 = r[s][2 * q + 1];
+                const unsigned hp = pack_bf16x2(x0, x1);
+                h[q] = hp;
+                if constexpr (SPLIT)
+                    l[q] = pack_bf16x2(x0 - __uint_as_float(hp << 16), x1 - __uint_as_float(hp & 0xffff0000u));
+            }
+            const int off = row * PK + (g0 + s * GS) * 8;
+            *reinterpret_cast<u4_t*>(hi_plane + off) = h;
+            if constexpr (SPLIT) *reinterpret_cast<u4_t*>(lo_plane + off) = l;
         }
-        for (int c = 0; c < nchunks; c++) {
-            __syncthreads();
-#pragma unroll
-            for (int s = 0; s < GA; s++) {
-                const int off = a_row * PK + (a_g + s * (NTHREADS / BM)) * 8;
-                *reinterpret_cast<s8_t*>(As + off) = ra[s];
-                if constexpr (SPLIT) *reinterpret_cast<s8_t*>(Al + off) = la[s];
-            }
-#pragma unroll
-            for (int s = 0; s < GB; s++) {
-                const int off = b_row * PK + (b_g + s * (NTHREADS / BN)) * 8;
-                *reinterpret_cast<s8_t*>(Bs + off) = rb[s];
-                if constexpr (SPLIT) *reinterpret_cast<s8_t*>(Bl + off) = lb[s];
-            }
-            __syncthreads();
-            if (c + 1 < nchunks) {
-                const int k0 = (c + 1) * BK;
-                load8<GA, NTHREADS / BM>(ra, la, Ab, lda, a_ok, k0 + a_g * 8, K);
-                load8<GB, NTHREADS / BN>(rb, lb, Bb, ldb, b_ok, k0 + b_g * 8, K);
-            }
-            const int kg = (lane >> 4) * 8;
+    }
+
+    static __device__ __forceinline__ void compute(acc_t (&acc)[WM][WN], const unsigned short* __restrict__ S,
+                                                   int lane, int wr, int wc) {
+        const unsigned short* As = S;
+        const unsigned short* Bs = As + BM * PK;
+        const unsigned short* Al = Bs + BN * PK;
+        const unsigned short* Bl = Al + BM * PK;
+#pragma unroll 1  // one k-step's hi/lo operands live at a time (register budget of the executors)
+        for (int ks = 0; ks < BK / 32; ks++) {
+            const int kg = ks * 32 + (lane >> 4) * 8;
             s8_t av[WM], bv[WN];
 #pragma unroll
             for (int a = 0; a < WM; a++)
@@ -272,9 +288,53 @@ struct GemmNTbf16 {
                     }
             }
         }
+    }
+
+    // Starts and ends with a workgroup barrier: the caller may use `lds` before and after.
+    static __device__ __forceinline__ void run(acc_t (&acc)[WM][WN], const TA* __restrict__ A, long lda, int mv,
+                               const TA* __restrict__ B, long ldb, int nv, int K, char* lds) {
+        unsigned short* S0 = reinterpret_cast<unsigned short*>(lds);
+        const int tid = threadIdx.x;
+        const int lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
+        // thread -> fixed row (tid % BM) and k-groups of 8 (tid / BM + s * NTHREADS / BM)
+        const int a_row = tid % BM, a_g = tid / BM, b_row = tid % BN, b_g = tid / BN;
+        const bool a_ok = a_row < mv, b_ok = b_row < nv;
+        const TA* Ab = A + a_row;
+        const TA* Bb = B + b_row;
+        const int nchunks = (K + BK - 1) / BK;
         __syncthreads();
+        if (nchunks == 0) return;
+        float ra[GA][8], rb[GB][8];
+        auto stage = [&](unsigned short* S) {
+            store_planes<GA, NTHREADS / BM>(S, S + (BM + BN) * PK, ra, a_row, a_g);
+            store_planes<GB, NTHREADS / BN>(S + BM * PK, S + (BM + BN) * PK + BM * PK, rb, b_row, b_g);
+        };
+        load_raw<GA, NTHREADS / BM>(ra, Ab, lda, a_ok, a_g * 8, K);
+        load_raw<GB, NTHREADS / BN>(rb, Bb, ldb, b_ok, b_g * 8, K);
+        stage(S0);
+        if (nchunks > 1) {
+            load_raw<GA, NTHREADS / BM>(ra, Ab, lda, a_ok, BK + a_g * 8, K);
+            load_raw<GB, NTHREADS / BN>(rb, Bb, ldb, b_ok, BK + b_g * 8, K);
+        }
+        __syncthreads();
+        for (int c = 0; c < nchunks; c++) {
+            compute(acc, S0 + (c & 1) * STAGE, lane, wr, wc);
+            if (c + 1 < nchunks) {
+                stage(S0 + ((c + 1) & 1) * STAGE);
+                if (c + 2 < nchunks) {
+                    const int k0 = (c + 2) * BK;
+                    load_raw<GA, NTHREADS / BM>(ra, Ab, lda, a_ok, k0 + a_g * 8, K);
+                    load_raw<GB, NTHREADS / BN>(rb, Bb, ldb, b_ok, k0 + b_g * 8, K);
+                }
+            }
+            __syncthreads();
+        }
     }
 };
+
+// K-chunk of the 128x128 GEMM items: 32 for FP64 (8 k-steps of 64-cycle MFMAs per
+// chunk), 64 for FP32 (the MFMA is twice as fast); both fill 130 KB of LDS.
+template <typename TC> constexpr int gemm_bk() { return sizeof(TC) == 8 ? 32 : 64; }
 
 // ---------------------------------------------------------------------------
 // GEMM / SYRK work item: C[r0:r0+128, c0:c0+128] = beta*C + alpha * A[r0:,:] B[c0:,:]^T
@@ -308,7 +368,7 @@ __device__ __forceinline__ void item_gemm(const TS* __restrict__ A, long lda, co
                 }
     } else {
         typedef TS TC;  // F64 tiles on FP64 MFMA, F32 tiles on FP32 MFMA
-        using G = GemmNT<TS, TS, TC, GEMM_BM, GEMM_BM, 16>;
+        using G = GemmNT<TS, TS, TC, GEMM_BM, GEMM_BM, gemm_bk<TC>()>;
         typename G::acc_t acc[G::WM][G::WN];
         G::zero(acc);
         G::run(acc, A + r0, lda, mv, B + c0, ldb, nv, K, lds);
@@ -343,136 +403,215 @@ __device__ __forceinline__ void decode_gemm_item(int it, int tr, bool syrk, int&
 }
 
 // ---------------------------------------------------------------------------
-// 64x64 diagonal block in LDS (column-major, ld DP): factor (lower Cholesky) and
-// invert in place. FP64 throughout. Padding rows/cols beyond `w` hold identity.
+// 64x64 diagonal block in LDS (column-major, ld DP), blocked by 16: factor
+// (lower Cholesky) and inverse. FP64 throughout. Padding rows/cols beyond `w`
+// hold identity. Diagonal 16x16 blocks are handled element-wise (thread t owns
+// element (t%16, t/16), one barrier per column); everything else is 16x16x16
+// products on FP64 MFMA, one block per wave.
 // ---------------------------------------------------------------------------
 constexpr int DP = DB + 1;
 
-// Right-looking Cholesky of D (w x w valid); one barrier per column. Returns the
-// first failing local column + 1 (0 if positive definite) to every thread.
-__device__ __forceinline__ int lds_potrf64(double* D, int w) {
-    const int tid = threadIdx.x;
-    __shared__ int s_bad;
-    if (tid == 0) s_bad = 0;
+// acc += P Q (q_trans = false) or P Q^T (q_trans = true); P, Q 16x16 blocks in LDS, ld DP.
+__device__ __forceinline__ d4_t mma16(d4_t acc, const double* P, const double* Q, bool q_trans) {
+    const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const int k = 4 * s + kq;
+        const double b = q_trans ? Q[m + k * DP] : Q[k + m * DP];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(P[m + k * DP], b, acc, 0, 0, 0);
+    }
+    return acc;
+}
+// R = sign * acc (acc in the FP64 MFMA result layout), or R += sign * acc.
+__device__ __forceinline__ void put16(double* R, d4_t acc, double sign, bool add) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        double* p = R + ((lane >> 4) + 4 * r) + (lane & 15) * DP;
+        *p = add ? *p + sign * acc[r] : sign * acc[r];
+    }
+}
+// out = P T with T held in MFMA result registers (its row 4s + lane/16 is acc[s]: the B operand layout).
+__device__ __forceinline__ d4_t mma16_reg(const double* P, d4_t T) {
+    const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
+    d4_t out = {0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < 4; s++) out = __builtin_amdgcn_mfma_f64_16x16x4f64(P[m + (4 * s + kq) * DP], T[s], out, 0, 0, 0);
+    return out;
+}
+
+// X_KK = L_KK^{-1} for the 16x16 lower-triangular diagonal block at Lk (X block at Xk,
+// upper part left untouched). Row-by-row substitution with the partial sums in registers:
+// one barrier per row. All threads participate.
+__device__ __forceinline__ void tri_inv16(const double* Lk, double* Xk) {
+    const int ti = threadIdx.x & 15, tc = threadIdx.x >> 4;
+    double acc = 0.0;
+    for (int q = 0; q < 16; q++) {
+        if (ti == q && tc <= q) Xk[q + tc * DP] = ((q == tc ? 1.0 : 0.0) - acc) / Lk[q + q * DP];
+        __syncthreads();
+        if (ti > q && tc <= q) acc += Lk[ti + q * DP] * Xk[q + tc * DP];
+    }
     __syncthreads();
-    for (int k = 0; k < w; k++) {
-        const double dkk = D[k + k * DP];
-        if (tid == 0 && !(dkk > 0.0) && s_bad == 0) s_bad = k + 1;
-        const double inv = 1.0 / dkk;
-        // trailing lower part: rows i in (k, w), cols j in (k, i]
-        const int t = w - k - 1;
-        for (int e = tid; e < t * t; e += NTHREADS) {
-            const int i = k + 1 + e % t, j = k + 1 + e / t;
-            if (i >= j) D[i + j * DP] -= D[i + k * DP] * D[j + k * DP] * inv;
+}
+
+// Off-diagonal blocks of X = L^{-1} from L (D) and the diagonal inverses already in X:
+// X_IJ = -X_II sum_{Q=J}^{I-1} L_IQ X_QJ, in order of I - J (one wave per block).
+__device__ __forceinline__ void tri_inv_offdiag(const double* D, double* X) {
+    const int wv = threadIdx.x >> 6;
+    for (int d = 1; d < 4; d++) {
+        const int J = wv, I = wv + d;
+        if (I < 4) {
+            d4_t T = {0, 0, 0, 0};
+            for (int Q = J; Q < I; Q++) T = mma16(T, D + 16 * I + 16 * Q * DP, X + 16 * Q + 16 * J * DP, false);
+            put16(X + 16 * I + 16 * J * DP, mma16_reg(X + 16 * I + 16 * I * DP, T), -1.0, false);
         }
         __syncthreads();
     }
-    // scale: L(:,k) = D(:,k) / sqrt(D(k,k))
-    __shared__ double s_sq[DB];
-    for (int k = tid; k < w; k += NTHREADS) s_sq[k] = sqrt(D[k + k * DP]);
+}
+
+// Factor D in place (lower; the upper part is left undefined) and write X = L^{-1}
+// (lower, upper part zero). Returns the first failing local column + 1 (0 if
+// positive definite) to every thread.
+__device__ __forceinline__ int lds_potrf_inv64(double* D, double* X, int w) {
+    const int tid = threadIdx.x, wv = tid >> 6;
+    const int ti = tid & 15, tj = tid >> 4;
+    __shared__ int s_bad;
+    if (tid == 0) s_bad = 0;
+    for (int e = tid; e < DB * DP; e += NTHREADS) X[e] = 0.0;
     __syncthreads();
-    for (int e = tid; e < w * w; e += NTHREADS) {
-        const int i = e % w, k = e / w;
-        if (i > k) D[i + k * DP] /= s_sq[k];
-        else if (i == k) D[k + k * DP] = s_sq[k];
+    for (int K = 0; K < 4; K++) {
+        const int k0 = 16 * K;
+        double* Dk = D + k0 + k0 * DP;
+        // diagonal block: unscaled right-looking elimination, then column scaling
+        for (int k = 0; k < 16; k++) {
+            const double dkk = Dk[k + k * DP];
+            if (tid == 0 && !(dkk > 0.0) && s_bad == 0 && k0 + k < w) s_bad = k0 + k + 1;
+            if (ti > k && tj > k && ti >= tj) Dk[ti + tj * DP] -= Dk[ti + k * DP] * Dk[tj + k * DP] / dkk;
+            __syncthreads();
+        }
+        const double sj = sqrt(Dk[tj + tj * DP]);
+        const double v = Dk[ti + tj * DP];
+        __syncthreads();
+        if (ti > tj) Dk[ti + tj * DP] = v / sj;
+        else if (ti == tj) Dk[ti + tj * DP] = sj;
+        __syncthreads();
+        tri_inv16(Dk, X + k0 + k0 * DP);
+        if (K == 3) break;
+        // panel: L_IK = D_IK X_KK^T (one wave per block below)
+        {
+            const int I = K + 1 + wv;
+            if (I < 4) {
+                double* Dik = D + 16 * I + k0 * DP;
+                const d4_t acc = mma16(d4_t{0, 0, 0, 0}, Dik, X + k0 + k0 * DP, true);
+                put16(Dik, acc, 1.0, false);
+            }
+        }
+        __syncthreads();
+        // trailing update: D_IJ -= L_IK L_JK^T for K < J <= I < 4
+        {
+            const int nblk = (3 - K) * (4 - K) / 2;
+            for (int q = wv; q < nblk; q += 4) {
+                int J = K + 1, rem = q;
+                while (rem >= 4 - J) { rem -= 4 - J; J++; }
+                const int I = J + rem;
+                const d4_t acc = mma16(d4_t{0, 0, 0, 0}, D + 16 * I + k0 * DP, D + 16 * J + k0 * DP, true);
+                put16(D + 16 * I + 16 * J * DP, acc, -1.0, true);
+            }
+        }
+        __syncthreads();
     }
-    __syncthreads();
+    tri_inv_offdiag(D, X);
     return s_bad;
 }
 
-// In-place inverse of the lower-triangular w x w L held in D (LAPACK dtrti2 order:
-// columns from last to first, each a triangular matrix-vector product with the
-// already inverted trailing block). Upper part is zeroed; padding becomes identity.
-__device__ __forceinline__ void lds_trtri64(double* D, int w) {
+// In-place style inverse of a lower-triangular 64x64 L held in D (w valid, identity
+// padding set here): X = L^{-1}, lower, upper part zero.
+__device__ __forceinline__ void lds_trtri64(double* D, double* X, int w) {
     const int tid = threadIdx.x;
     for (int e = tid; e < DB * DB; e += NTHREADS) {
         const int i = e % DB, j = e / DB;
-        if (i < j) D[i + j * DP] = 0.0;
-        else if (i >= w || j >= w) D[i + j * DP] = (i == j) ? 1.0 : 0.0;
+        if (i >= w || j >= w) D[i + j * DP] = (i == j) ? 1.0 : 0.0;
     }
+    for (int e = tid; e < DB * DP; e += NTHREADS) X[e] = 0.0;
     __syncthreads();
-    for (int j = w - 1; j >= 0; j--) {
-        const double ajj = 1.0 / D[j + j * DP];
-        // x_i = sum_{q=j+1}^{i} Xinv(i,q) * L(q,j) for i > j
-        double x = 0.0;
-        const int i = j + 1 + tid;
-        if (tid < w - j - 1) {
-            for (int q = j + 1; q <= i; q++) x += D[i + q * DP] * D[q + j * DP];
-        }
-        __syncthreads();
-        if (tid < w - j - 1) D[i + j * DP] = -ajj * x;
-        if (tid == 0) D[j + j * DP] = ajj;
-        __syncthreads();
-    }
+    for (int K = 0; K < 4; K++) tri_inv16(D + 16 * K + 16 * K * DP, X + 16 * K + 16 * K * DP);
+    tri_inv_offdiag(D, X);
 }
 
 // ---------------------------------------------------------------------------
 // POTRF work item: the whole n x n diagonal tile in one workgroup, left-looking
-// over 64-column blocks; leaves inverses of the 64x64 diagonal blocks of L in dinv.
-// Arithmetic is FP64 whatever the tile storage (the mixed variants keep the
-// diagonal factorization in FP64).
+// over 64-column panels. Per panel: one tall GEMM updates the panel rows p0..n
+// (256 rows per call), the 64x64 diagonal block is factored and inverted in LDS
+// (lds_potrf_inv64), and the rows below are multiplied by the inverse's transpose.
+// Leaves the inverses of the 64x64 diagonal blocks of L in dinv. Arithmetic is
+// FP64 whatever the tile storage (the mixed variants keep the diagonal
+// factorization in FP64).
 // ---------------------------------------------------------------------------
+constexpr int POTRF_BM = 128;  // rows per panel GEMM call
+template <typename TS>
+struct PotrfGemm {
+    using G = GemmNT<TS, TS, double, POTRF_BM, DB, 32>;        // panel update
+    using G2 = GemmNT<TS, double, double, POTRF_BM, DB, 32>;   // times the inverse
+};
+
 template <typename TS>
 __device__ __forceinline__ void item_potrf(TS* __restrict__ A, long lda, int n, double* __restrict__ dinv, int* info, int col_off,
                            char* lds) {
-    using G = GemmNT<TS, TS, double, 64, 64, 32>;
-    using G2 = GemmNT<TS, double, double, 64, 64, 32>;
-    static_assert(G::LDS_BYTES == G2::LDS_BYTES, "staging size");
-    double* D = reinterpret_cast<double*>(lds + G::LDS_BYTES);
+    using G = typename PotrfGemm<TS>::G;
+    using G2 = typename PotrfGemm<TS>::G2;
+    double* D = reinterpret_cast<double*>(lds);  // shares the GEMM staging area (used between GEMM calls)
+    double* X = D + DB * DP;
     const int tid = threadIdx.x;
     const int nblk = (n + DB - 1) / DB;
     for (int pb = 0; pb < nblk; pb++) {
-        const int p0 = pb * DB, pw = min(DB, n - p0);
-        // D = A(p,p) - L(p, 0:p0) L(p, 0:p0)^T
+        const int p0 = pb * DB, pw = min(DB, n - p0), rows = n - p0;
         typename G::acc_t acc[G::WM][G::WN];
-        G::zero(acc);
-        if (p0 > 0) G::run(acc, A + p0, lda, pw, A + p0, lda, pw, p0, lds);
+        // panel update: A(p0:n, p) -= L(p0:n, 0:p0) L(p, 0:p0)^T
+        if (p0 > 0) {
+            for (int r0 = 0; r0 < rows; r0 += POTRF_BM) {
+                const int mv = min(POTRF_BM, rows - r0);
+                TS* T = A + (p0 + r0) + (long)p0 * lda;
+                G::zero(acc);
+                G::run(acc, A + p0 + r0, lda, mv, A + p0, lda, pw, p0, lds);
 #pragma unroll
-        for (int a = 0; a < G::WM; a++)
+                for (int a = 0; a < G::WM; a++)
 #pragma unroll
-            for (int b = 0; b < G::WN; b++)
+                    for (int b = 0; b < G::WN; b++)
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int rr = G::row_of(a, r), cc = G::col_of(b);
-                    double v;
-                    if (rr < pw && cc < pw) v = (rr >= cc) ? (double)A[(p0 + rr) + (long)(p0 + cc) * lda] - acc[a][b][r] : 0.0;
-                    else v = (rr == cc) ? 1.0 : 0.0;
-                    D[rr + cc * DP] = v;
-                }
-        __syncthreads();
-        const int bad = lds_potrf64(D, pw);
-        if (bad && tid == 0) atomicCAS(info, 0, col_off + p0 + bad);
-        // write L(p,p) (lower) back to the tile
-        for (int e = tid; e < pw * pw; e += NTHREADS) {
-            const int i = e % pw, j = e / pw;
-            if (i >= j) A[(p0 + i) + (long)(p0 + j) * lda] = (TS)D[i + j * DP];
-        }
-        __syncthreads();
-        lds_trtri64(D, pw);
-        double* Xinv = dinv + (long)pb * DB * DB;
-        for (int e = tid; e < DB * DB; e += NTHREADS) Xinv[e] = D[(e % DB) + (e / DB) * DP];
-        __syncthreads();
-        // panel below: L(q,p) = (A(q,p) - L(q,0:p0) L(p,0:p0)^T) Xinv^T
-        for (int qb = pb + 1; qb < nblk; qb++) {
-            const int q0 = qb * DB, qw = min(DB, n - q0);
-            TS* T = A + q0 + (long)p0 * lda;
-            G::zero(acc);
-            if (p0 > 0) G::run(acc, A + q0, lda, qw, A + p0, lda, pw, p0, lds);
-#pragma unroll
-            for (int a = 0; a < G::WM; a++)
-#pragma unroll
-                for (int b = 0; b < G::WN; b++)
-#pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const int rr = G::row_of(a, r), cc = G::col_of(b);
-                        if (rr < qw && cc < pw) {
-                            TS* p = T + rr + (long)cc * lda;
-                            *p = (TS)((double)(*p) - acc[a][b][r]);
+                        for (int r = 0; r < 4; r++) {
+                            const int rr = G::row_of(a, r), cc = G::col_of(b);
+                            if (rr < mv && cc < pw && (r0 + rr >= cc)) {
+                                TS* q = T + rr + (long)cc * lda;
+                                *q = (TS)((double)(*q) - acc[a][b][r]);
+                            }
                         }
-                    }
+            }
             __syncthreads();
+        }
+        // diagonal block into LDS (identity padding), factor + inverse
+        for (int e = tid; e < DB * DB; e += NTHREADS) {
+            const int i = e % DB, j = e / DB;
+            double v;
+            if (i < pw && j < pw) v = (i >= j) ? (double)A[(p0 + i) + (long)(p0 + j) * lda] : 0.0;
+            else v = (i == j) ? 1.0 : 0.0;
+            D[i + j * DP] = v;
+        }
+        __syncthreads();
+        const int bad = lds_potrf_inv64(D, X, pw);
+        if (bad && tid == 0) atomicCAS(info, 0, col_off + p0 + bad);
+        double* Xinv = dinv + (long)pb * DB * DB;
+        for (int e = tid; e < DB * DB; e += NTHREADS) {
+            const int i = e % DB, j = e / DB;
+            if (i < pw && j < pw && i >= j) A[(p0 + i) + (long)(p0 + j) * lda] = (TS)D[i + j * DP];
+            Xinv[e] = X[i + j * DP];
+        }
+        __syncthreads();
+        // rows below: L(q, p) = A(q, p) Xinv^T
+        for (int r0 = DB; r0 < rows; r0 += POTRF_BM) {
+            const int mv = min(POTRF_BM, rows - r0);
+            TS* T = A + (p0 + r0) + (long)p0 * lda;
             G2::zero(acc);
-            G2::run(acc, T, lda, qw, Xinv, DB, pw, pw, lds);
+            G2::run(acc, T, lda, mv, Xinv, DB, pw, pw, lds);
 #pragma unroll
             for (int a = 0; a < G::WM; a++)
 #pragma unroll
@@ -480,15 +619,15 @@ __device__ __forceinline__ void item_potrf(TS* __restrict__ A, long lda, int n, 
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
                         const int rr = G::row_of(a, r), cc = G::col_of(b);
-                        if (rr < qw && cc < pw) T[rr + (long)cc * lda] = (TS)acc[a][b][r];
+                        if (rr < mv && cc < pw) T[rr + (long)cc * lda] = (TS)acc[a][b][r];
                     }
-            __syncthreads();
         }
+        __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
-// TRSM work item: rows [r0, r0+64) of X (m x n) := X L^{-T} (Right, Lower,
+// TRSM work item: rows [r0, r0+TRSM_BM) of X (m x n) := X L^{-T} (Right, Lower,
 // Trans, NonUnit), in place, by 64-column blocks: subtract the already solved
 // columns times L's off-diagonal blocks, then multiply by the inverse of L's
 // 64x64 diagonal block left by POTRF. TC is the MFMA type (FP64, or FP32 for the
@@ -497,8 +636,8 @@ __device__ __forceinline__ void item_potrf(TS* __restrict__ A, long lda, int n, 
 template <typename TS, typename TC>
 __device__ __forceinline__ void item_trsm(const TS* __restrict__ L, long ldl, const double* __restrict__ dinv, TS* __restrict__ X,
                           long ldx, int m, int n, int r0, char* lds) {
-    using G = GemmNT<TS, TS, TC, 64, 64, 32>;
-    using G2 = GemmNT<TS, double, TC, 64, 64, 32>;
+    using G = GemmNT<TS, TS, TC, TRSM_BM, DB, 32>;
+    using G2 = GemmNT<TS, double, TC, TRSM_BM, DB, 32>;
     const int mv = min(TRSM_BM, m - r0);
     if (mv <= 0) return;
     TS* Xr = X + r0;
@@ -538,11 +677,12 @@ __device__ __forceinline__ void item_trsm(const TS* __restrict__ L, long ldl, co
 }
 
 // LDS bytes each item kind needs (the executor reserves the maximum).
-constexpr int LDS_GEMM_F64 = GemmNT<double, double, double, GEMM_BM, GEMM_BM, 16>::LDS_BYTES;
-constexpr int LDS_GEMM_F32 = GemmNT<float, float, float, GEMM_BM, GEMM_BM, 16>::LDS_BYTES;
+constexpr int LDS_GEMM_F64 = GemmNT<double, double, double, GEMM_BM, GEMM_BM, gemm_bk<double>()>::LDS_BYTES;
+constexpr int LDS_GEMM_F32 = GemmNT<float, float, float, GEMM_BM, GEMM_BM, gemm_bk<float>()>::LDS_BYTES;
 constexpr int LDS_GEMM_BF16 = GemmNTbf16<float, GEMM_BM, GEMM_BM, true>::LDS_BYTES;
-constexpr int LDS_POTRF = GemmNT<double, double, double, 64, 64, 32>::LDS_BYTES + DB * DP * 8;
-constexpr int LDS_TRSM = GemmNT<double, double, double, 64, 64, 32>::LDS_BYTES;
+constexpr int LDS_POTRF = PotrfGemm<double>::G::LDS_BYTES > 2 * DB * DP * 8 ? PotrfGemm<double>::G::LDS_BYTES
+                                                                             : 2 * DB * DP * 8;
+constexpr int LDS_TRSM = GemmNT<double, double, double, TRSM_BM, DB, 32>::LDS_BYTES;
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 constexpr int LDS_ITEM_MAX = cmax(cmax(cmax(LDS_GEMM_F64, LDS_GEMM_F32), cmax(LDS_GEMM_BF16, LDS_POTRF)), LDS_TRSM);
 
