@@ -60,17 +60,17 @@ def make_matrix(n: int, seed_b: int, seed_d: int, delta: float, scaled: bool):
     return A  # symmetric, so row-major storage equals column-major
 
 
-def cpu_baseline(n_sample: int, b: int, threads: int):
+def cpu_baseline(n_sample: int, b: int, threads: int, prio: int = 0):
     """The oracle (Taskflow+Threadpool restatement, sequential MKL per task) on the host cores."""
     import numpy as np
     from oracle import oracle as orc
     orc.build()
     kind = orc.set_backend("mkl") if orc.have_mkl() else orc.set_backend("textbook")
     A = orc.spd_bbt(n_sample, 2020)
-    orc.cholesky(A, b, n_threads=threads, prio_kind=2)  # warm-up, as 2d_cholesky.cpp:54-61
+    orc.cholesky(A, b, n_threads=threads, prio_kind=prio)  # warm-up, as 2d_cholesky.cpp:54-61
     times = []
     for _ in range(3):
-        _, st = orc.cholesky(A, b, n_threads=threads, prio_kind=2)
+        _, st = orc.cholesky(A, b, n_threads=threads, prio_kind=prio)
         times.append(st["total"])
     t = float(np.median(times))
     return {"value": round(flops(n_sample) / t / 1e12, 4), "unit": "TFLOP/s", "cores": threads, "kind": "port",
@@ -182,6 +182,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson request rate of the serving leg (1/s)")
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--prio", type=int, default=0,
+                    help="PrioKind of the FP64 leg and the CPU baseline (the mixed legs use 0, the miniapp default)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -214,7 +216,7 @@ def main():
     def fp64_step():
         t0 = time.perf_counter()
         W64.copy_from(A64)
-        info = ctx.potrf_tiled(W64, tt.TTAD_SCHED_PERSISTENT, 2, "f64")
+        info = ctx.potrf_tiled(W64, tt.TTAD_SCHED_PERSISTENT, args.prio, "f64")
         if info != 0:
             raise RuntimeError(f"FP64 factorization failed, info={info}")
         t_fac = ctx.last_elapsed()
@@ -300,7 +302,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{workload}: Cholesky + FP64 refinement to eta<=1e-12, n={n} b={b} nrhs={nrhs}, "
                                f"A=D(BB^T/n+1e-2 I)D",
-                   "n": n, "tile": b, "nrhs": nrhs, "prio_kind": 2, "schedule": "persistent"},
+                   "n": n, "tile": b, "nrhs": nrhs, "prio_kind": args.prio, "schedule": "persistent"},
         "legs": res,
         "roofline": {"bound": "mfma", "achieved": round(F / t_fac64 / 1e12, 3), "peak": PEAK["f64"],
                      "unit": "TFLOP/s", "frac": round(F / t_fac64 / 1e12 / PEAK["f64"], 4), "traffic": None,
@@ -315,7 +317,7 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline: oracle n={args.cpu_n} b={b} threads={threads}")
-        out["cpu_baseline"] = cpu_baseline(args.cpu_n, b, threads)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n, b, threads, args.prio)
     print(json.dumps(out), flush=True)
 
 

@@ -124,8 +124,9 @@ int ttad_potrf_tiled(ttad_context* ctx, ttad_matrix* M, int schedule, int prio_k
                      int* info);
 
 /* Device time (HIP events on the factorization's stream) of the last
- * ttad_potrf_tiled on this context; replaces the t0/t1 wall-clock window of
- * 2d_cholesky.cpp:611-619. */
+ * ttad_potrf_tiled on this context: the persistent executor's launch for
+ * TTAD_SCHED_PERSISTENT, all launches for TTAD_SCHED_STREAM; replaces the t0/t1
+ * wall-clock window of 2d_cholesky.cpp:611-619. */
 int ttad_last_elapsed(ttad_context* ctx, double* seconds);
 
 /* ---- device task graphs: the Taskflow<K>/Threadpool drop-in ------------------- */
@@ -141,7 +142,7 @@ typedef struct {
     long long items_done;
     long long items_total;
     int workers;            /* persistent workgroups */
-    int abort_code;         /* 0 ok, 1 watchdog, 2 queue overflow */
+    int abort_code;         /* 0 ok, 1 watchdog, 2 queue overflow, 3 item variant missing from the executor */
 } ttad_graph_stats;
 
 /* A graph is the compiled form of the Taskflow callbacks (taskflow.hpp:40-297):

@@ -105,7 +105,10 @@ int cholesky_persistent(ttad_context* ctx, ttad_matrix* M, int prio_kind, int pr
     if (rc != TTAD_OK) return rc;
     rc = ttad_graph_launch(g, 0, 0, s);
     if (rc != TTAD_OK) return rc;
-    return ttad_graph_wait(g, 120.0, nullptr);
+    ttad_graph_stats st{};
+    rc = ttad_graph_wait(g, 120.0, &st);
+    if (rc == TTAD_OK) ctx->kernel_s = st.elapsed_s;
+    return rc;
 }
 
 // The PTG compiled on the host (no device), for graphs that embed it (serving).

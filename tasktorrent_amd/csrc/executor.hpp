@@ -39,6 +39,9 @@ __device__ __forceinline__ int ld_relaxed(const int32_t* p) {
 __device__ __forceinline__ int64_t ld_acquire64(const int64_t* p) {
     return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ int64_t ld_relaxed64(const int64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Item variants a kernel is compiled with (bit mask). A kernel holding every
 // variant allocates registers for the largest one; the executors are therefore
@@ -66,6 +69,20 @@ inline int vset_index(int needed) {
     return N_VSETS - 1;
 }
 
+// POTRF and TRSM items are called out of line from the executors: a kernel holding them
+// inline is register-allocated for their largest live set and spills in the GEMM loop,
+// which runs far more often. The call saves the callee's registers once per item.
+template <typename TS>
+__device__ __attribute__((noinline)) void potrf_call(TS* A, long lda, int n, double* dinv, int* info, int col_off,
+                                                     char* lds) {
+    item_potrf<TS>(A, lda, n, dinv, info, col_off, lds);
+}
+template <typename TS, typename TC>
+__device__ __attribute__((noinline)) void trsm_call(const TS* L, long ldl, const double* dinv, TS* X, long ldx, int m,
+                                                    int n, int r0, char* lds) {
+    item_trsm<TS, TC>(L, ldl, dinv, X, ldx, m, n, r0, lds);
+}
+
 // Execute work item `it` of task t. All 256 threads participate. Returns false
 // (doing nothing) for an item variant outside VM.
 template <int VM>
@@ -79,12 +96,12 @@ __device__ __forceinline__ bool exec_item(const DevTask& t, int it, const DevMat
             const int j = t.j, n = M.bsz[j];
             if constexpr ((VM & V_F64S) != 0)
                 if (f64s) {
-                    item_potrf<double>((double*)tile(j, j), n, n, M.dinv[j], M.info, M.boff[j], lds);
+                    potrf_call<double>((double*)tile(j, j), n, n, M.dinv[j], M.info, M.boff[j], lds);
                     return true;
                 }
             if constexpr ((VM & V_F32S) != 0)
                 if (!f64s) {
-                    item_potrf<float>((float*)tile(j, j), n, n, M.dinv[j], M.info, M.boff[j], lds);
+                    potrf_call<float>((float*)tile(j, j), n, n, M.dinv[j], M.info, M.boff[j], lds);
                     return true;
                 }
             return false;
@@ -93,13 +110,13 @@ __device__ __forceinline__ bool exec_item(const DevTask& t, int it, const DevMat
             const int i = t.i, j = t.j, m = M.bsz[i], n = M.bsz[j];
             if constexpr ((VM & V_F64S) != 0)
                 if (f64s) {
-                    item_trsm<double, double>((const double*)tile(j, j), n, M.dinv[j], (double*)tile(i, j), m, m, n,
+                    trsm_call<double, double>((const double*)tile(j, j), n, M.dinv[j], (double*)tile(i, j), m, m, n,
                                               it * TRSM_BM, lds);
                     return true;
                 }
             if constexpr ((VM & V_F32S) != 0)
                 if (!f64s) {
-                    item_trsm<float, float>((const float*)tile(j, j), n, M.dinv[j], (float*)tile(i, j), m, m, n,
+                    trsm_call<float, float>((const float*)tile(j, j), n, M.dinv[j], (float*)tile(i, j), m, m, n,
                                             it * TRSM_BM, lds);
                     return true;
                 }

@@ -26,17 +26,20 @@ __global__ __launch_bounds__(NTHREADS) void k_items(const DevMat* mats, const De
 }
 
 // ---------------------------------------------------------------------------
-// Persistent DAG executor. Every workgroup loops:
-//   wave 0 scans the priority buckets (one bucket per lane), claims the head
-//   entry of the highest-priority non-empty bucket with a CAS, the workgroup runs
-//   that work item, then thread 0 decrements the task's remaining-items counter;
-//   the last item of a task decrements its successors' dependency counters and
-//   pushes every item of a successor that reaches zero into its bucket.
+// Persistent DAG executor (replaces the Threadpool worker loop). Every workgroup loops:
+//   wave 0 reads the heads/tails of the priority buckets 64 at a time (one per lane,
+//   <= 256 buckets) and claims the head entry of the highest-priority non-empty bucket:
+//   with a fetch-add when the bucket holds more entries than there are workers
+//   (no claim can then pass the tail), else with a CAS (losers back off a random
+//   interval); the workgroup runs that work item, then thread 0 decrements the
+//   task's remaining-items counter; the last item of a task decrements its
+//   successors' dependency counters and pushes every item of a successor that
+//   reaches zero into its bucket. A worker exits when every bucket is fully claimed.
 // Memory ordering: item outputs are released (agent scope) before the counter
 // RMWs; queue entries are published with release stores and read with acquire
 // loads, so a consumer on another XCD sees the producer's tiles.
-// The loop ends when all items are done or the abort flag is raised (watchdog:
-// no work found for spin_limit ticks; queue overflow).
+// The loop also ends when the abort flag is raised (watchdog: no work found for
+// spin_limit ticks; queue overflow; an item variant missing from this instantiation).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void push_task(const DevGraph& g, int s) {
     const DevTask t = g.tasks[s];
@@ -57,36 +60,52 @@ __global__ __launch_bounds__(NTHREADS) void k_persistent(DevGraph g) {
     extern __shared__ __align__(16) char lds[];
     __shared__ int64_t s_item;
     const int tid = threadIdx.x;
+    const int margin = 2 * (int)gridDim.x;  // more queued items than this: a fetch-add claim cannot overshoot
+    uint32_t rng = blockIdx.x * 2654435761u + 0x9e3779b9u;
     for (;;) {
         if (tid < 64) {
+            // wave 0: ballots over groups of 64 buckets (lane = bucket), highest priority first
             int64_t got = -1;
             const int64_t t_start = (int64_t)wall_clock64();
             for (;;) {
-                int fb = -1, fh = 0;
-                for (int base = 0; base < g.n_buckets; base += 64) {
+                int fb = -1, fh = 0, ft = 0;
+                bool any_open = false;
+                for (int base = 0; base < g.n_buckets; base += 64) {  // groups of 64 buckets, best first
                     const int b = base + tid;
-                    int h = 0;
-                    bool ne = false;
+                    int h = 0, t = 0;
+                    bool ne = false, open = false;
                     if (b < g.n_buckets) {
                         h = ld_relaxed(g.q_head + b);
-                        ne = h < ld_relaxed(g.q_tail + b);
+                        t = ld_relaxed(g.q_tail + b);
+                        ne = h < t;
+                        open = h < g.q_base[b + 1] - g.q_base[b];
                     }
                     const unsigned long long mask = __ballot(ne);
+                    any_open |= __ballot(open) != 0;
                     if (mask) {
                         const int l = __ffsll((long long)mask) - 1;
                         fb = base + l;
                         fh = __shfl(h, l);
+                        ft = __shfl(t, l);
                         break;
                     }
                 }
+                const bool all_claimed = fb < 0 && !any_open;
                 if (fb >= 0) {
                     int64_t e = -1;
                     if (tid == 0) {
-                        int h = fh;
-                        if (__hip_atomic_compare_exchange_strong(g.q_head + fb, &h, fh + 1, __ATOMIC_RELAXED,
-                                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                            const int64_t* slot = g.q_items + g.q_base[fb] + fh;
-                            while ((e = ld_acquire64(slot)) < 0) {
+                        int slot = -1;
+                        if (ft - fh > margin) {
+                            slot = __hip_atomic_fetch_add(g.q_head + fb, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        } else {
+                            int hh = fh;
+                            if (__hip_atomic_compare_exchange_strong(g.q_head + fb, &hh, fh + 1, __ATOMIC_RELAXED,
+                                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                                slot = fh;
+                        }
+                        if (slot >= 0) {
+                            const int64_t* sp = g.q_items + g.q_base[fb] + slot;
+                            while ((e = ld_acquire64(sp)) < 0) {  // claimed before its producer wrote it
                                 if ((int64_t)wall_clock64() - t_start > g.spin_limit) {
                                     __hip_atomic_store(g.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                                     e = -2;
@@ -101,9 +120,12 @@ __global__ __launch_bounds__(NTHREADS) void k_persistent(DevGraph g) {
                         got = e;
                         break;
                     }
-                    continue;  // lost the race for that slot: rescan
+                    // lost the CAS: back off a random 0-15 x 64 cycles so the losers do not retry in lockstep
+                    rng = rng * 1664525u + 1013904223u;
+                    for (uint32_t q = rng >> 28; q > 0; q--) __builtin_amdgcn_s_sleep(1);
+                    continue;
                 }
-                if (ld_relaxed(g.done_items) >= g.total_items || ld_relaxed(g.abort_flag) != 0) {
+                if (all_claimed || ld_relaxed(g.abort_flag) != 0) {  // every item is taken: this worker is done
                     got = -2;
                     break;
                 }
@@ -141,15 +163,15 @@ __global__ __launch_bounds__(NTHREADS) void k_persistent(DevGraph g) {
             }
             const int left =
                 __hip_atomic_fetch_add(g.items_left + task, -1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) - 1;
-            if (left == 0) {
+            if (left == 0) {  // last item of the task: release its successors
                 for (int q = g.out_ptr[task]; q < g.out_ptr[task + 1]; q++) {
                     const int s = g.out_idx[q];
                     const int d =
                         __hip_atomic_fetch_add(g.deps + s, -1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) - 1;
                     if (d == 0) push_task(g, s);
                 }
+                __hip_atomic_fetch_add(g.done_items, t.n_items, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            __hip_atomic_fetch_add(g.done_items, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
@@ -296,15 +318,21 @@ hipError_t kernels_init() {
     TTAD_SETLDS(k_persistent<VSETS[2]>);
     TTAD_SETLDS(k_persistent<VSETS[3]>);
     TTAD_SETLDS(k_persistent<VSETS[4]>);
-    TTAD_SETLDS((k_raw_gemm<double, P_F64>));
-    TTAD_SETLDS((k_raw_gemm<float, P_F32>));
-    TTAD_SETLDS((k_raw_gemm<float, P_BF16>));
-    TTAD_SETLDS((k_raw_gemm<float, P_BF16X3>));
-    TTAD_SETLDS((k_raw_potrf<double>));
-    TTAD_SETLDS((k_raw_potrf<float>));
-    TTAD_SETLDS((k_raw_trsm<double, double>));
-    TTAD_SETLDS((k_raw_trsm<float, float>));
 #undef TTAD_SETLDS
+#define TTAD_SETLDS2(fn, bytes)                      \
+    do {                                             \
+        e = set_lds((const void*)(fn), (bytes));     \
+        if (e != hipSuccess) return e;               \
+    } while (0)
+    TTAD_SETLDS2((k_raw_gemm<double, P_F64>), (gemm_item_lds<double, P_F64>()));
+    TTAD_SETLDS2((k_raw_gemm<float, P_F32>), (gemm_item_lds<float, P_F32>()));
+    TTAD_SETLDS2((k_raw_gemm<float, P_BF16>), (gemm_item_lds<float, P_BF16>()));
+    TTAD_SETLDS2((k_raw_gemm<float, P_BF16X3>), (gemm_item_lds<float, P_BF16X3>()));
+    TTAD_SETLDS2((k_raw_potrf<double>), LDS_POTRF);
+    TTAD_SETLDS2((k_raw_potrf<float>), LDS_POTRF);
+    TTAD_SETLDS2((k_raw_trsm<double, double>), LDS_TRSM);
+    TTAD_SETLDS2((k_raw_trsm<float, float>), LDS_TRSM);
+#undef TTAD_SETLDS2
     return e;
 }
 
@@ -365,25 +393,25 @@ void launch_raw_gemm(int prec, const void* A, long lda, const void* B, long ldb,
     const int items = syrk ? tr * (tr + 1) / 2 : tr * tc;
     if (items <= 0) return;
     if (prec == P_F64)
-        hipLaunchKernelGGL((k_raw_gemm<double, P_F64>), dim3(items), dim3(NTHREADS), LDS_ITEM_MAX, s,
+        hipLaunchKernelGGL((k_raw_gemm<double, P_F64>), dim3(items), dim3(NTHREADS), (gemm_item_lds<double, P_F64>()), s,
                            (const double*)A, lda, (const double*)B, ldb, (double*)C, ldc, m, n, K, syrk, alpha, beta);
     else if (prec == P_F32)
-        hipLaunchKernelGGL((k_raw_gemm<float, P_F32>), dim3(items), dim3(NTHREADS), LDS_ITEM_MAX, s, (const float*)A,
+        hipLaunchKernelGGL((k_raw_gemm<float, P_F32>), dim3(items), dim3(NTHREADS), (gemm_item_lds<float, P_F32>()), s, (const float*)A,
                            lda, (const float*)B, ldb, (float*)C, ldc, m, n, K, syrk, alpha, beta);
     else if (prec == P_BF16)
-        hipLaunchKernelGGL((k_raw_gemm<float, P_BF16>), dim3(items), dim3(NTHREADS), LDS_ITEM_MAX, s,
+        hipLaunchKernelGGL((k_raw_gemm<float, P_BF16>), dim3(items), dim3(NTHREADS), (gemm_item_lds<float, P_BF16>()), s,
                            (const float*)A, lda, (const float*)B, ldb, (float*)C, ldc, m, n, K, syrk, alpha, beta);
     else
-        hipLaunchKernelGGL((k_raw_gemm<float, P_BF16X3>), dim3(items), dim3(NTHREADS), LDS_ITEM_MAX, s,
+        hipLaunchKernelGGL((k_raw_gemm<float, P_BF16X3>), dim3(items), dim3(NTHREADS), (gemm_item_lds<float, P_BF16X3>()), s,
                            (const float*)A, lda, (const float*)B, ldb, (float*)C, ldc, m, n, K, syrk, alpha, beta);
 }
 
 void launch_raw_potrf(int prec, void* A, long lda, int n, double* dinv, int* info, hipStream_t s) {
     if (prec == P_F64)
-        hipLaunchKernelGGL((k_raw_potrf<double>), dim3(1), dim3(NTHREADS), LDS_ITEM_MAX, s, (double*)A, lda, n, dinv,
+        hipLaunchKernelGGL((k_raw_potrf<double>), dim3(1), dim3(NTHREADS), LDS_POTRF, s, (double*)A, lda, n, dinv,
                            info);
     else
-        hipLaunchKernelGGL((k_raw_potrf<float>), dim3(1), dim3(NTHREADS), LDS_ITEM_MAX, s, (float*)A, lda, n, dinv,
+        hipLaunchKernelGGL((k_raw_potrf<float>), dim3(1), dim3(NTHREADS), LDS_POTRF, s, (float*)A, lda, n, dinv,
                            info);
 }
 
@@ -400,10 +428,10 @@ void launch_raw_trsm(int prec, const void* L, long ldl, const double* dinv, void
     const int items = (m + TRSM_BM - 1) / TRSM_BM;
     if (items <= 0) return;
     if (prec == P_F64)
-        hipLaunchKernelGGL((k_raw_trsm<double, double>), dim3(items), dim3(NTHREADS), LDS_ITEM_MAX, s,
+        hipLaunchKernelGGL((k_raw_trsm<double, double>), dim3(items), dim3(NTHREADS), LDS_TRSM, s,
                            (const double*)L, ldl, dinv, (double*)X, ldx, m, n);
     else
-        hipLaunchKernelGGL((k_raw_trsm<float, float>), dim3(items), dim3(NTHREADS), LDS_ITEM_MAX, s, (const float*)L,
+        hipLaunchKernelGGL((k_raw_trsm<float, float>), dim3(items), dim3(NTHREADS), LDS_TRSM, s, (const float*)L,
                            ldl, dinv, (float*)X, ldx, m, n);
 }
 

@@ -68,6 +68,7 @@ struct ttad_context {
     long long next_matrix_id = 1;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;  // bracket the last ttad_potrf_tiled on its stream
     bool ev_valid = false;
+    double kernel_s = -1.0;  // event time of the last persistent factorization kernel (< 0: stream schedule)
     // solve-phase workspace (grown on demand)
     ttad::DevBuf sw_Y, sw_D, sw_R, sw_flags, sw_ticket, sw_err, sw_norms;
     // working copies of FP64 matrices for ttad_posv_mixed, keyed by (matrix id, storage)

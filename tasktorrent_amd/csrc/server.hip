@@ -21,16 +21,24 @@ namespace ttad {
 // ---------------------------------------------------------------------------
 constexpr int SRV_LDS = cmax(LDS_ITEM_MAX, sweep_lds_bytes<double, 32>());
 
-__device__ __forceinline__ int64_t srv_try_pop(const DevServer& s) {
-    // lane 0 only: claim the head of the first non-empty ring in priority order
+__device__ __forceinline__ int64_t srv_try_pop(const DevServer& s, uint32_t& rng) {
+    // lane 0 only: claim the head of the first non-empty ring in priority order: a fetch-add
+    // when the ring holds more entries than there are workers (the claim cannot pass the
+    // tail), else a CAS whose losers back off a random interval
+    const unsigned long long margin = 2ull * gridDim.x;
     for (int b = 0; b < s.n_buckets; b++) {
         const SrvRing& R = s.rings[b];
         unsigned long long h = __hip_atomic_load(R.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned long long t = __hip_atomic_load(R.tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (h >= t) continue;
-        if (!__hip_atomic_compare_exchange_strong(R.head, &h, h + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT))
+        if (t - h > margin) {
+            h = __hip_atomic_fetch_add(R.head, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else if (!__hip_atomic_compare_exchange_strong(R.head, &h, h + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT)) {
+            rng = rng * 1664525u + 1013904223u;
+            for (uint32_t q = rng >> 28; q > 0; q--) __builtin_amdgcn_s_sleep(1);
             return -1;  // lost the race: caller rescans
+        }
         const unsigned long long cap = 1ull << s.ring_log2;
         const unsigned long long lap = ((h >> s.ring_log2) + 1) & 0xFFFFFFull;
         const unsigned long long* slot = R.slots + (h & (cap - 1));
@@ -54,11 +62,12 @@ __global__ __launch_bounds__(NTHREADS) void k_server(DevServer s) {
     __shared__ int64_t s_item;
     const int tid = threadIdx.x;
     int64_t idle_since = (int64_t)wall_clock64();
+    uint32_t rng = blockIdx.x * 2654435761u + 0x9e3779b9u;
     for (;;) {
         if (tid == 0) {
             int64_t got = -1;
             for (;;) {
-                got = srv_try_pop(s);
+                got = srv_try_pop(s, rng);
                 if (got >= 0 || got == -2) break;
                 const int shut = __hip_atomic_load((const int32_t*)s.shutdown, __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_SYSTEM);

@@ -9,6 +9,11 @@
 #include "solve_ops.hpp"
 
 namespace ttad {
+// Priority kind of the factorizations inside ttad_posv_mixed: 0, the miniapp's default
+// (2d_cholesky.cpp `--kind`), only orders POTRF > TRSM > GEMM; with three buckets the GEMM
+// bucket always holds more items than there are workers, so claims are contention-free.
+constexpr int MIXED_PRIO_KIND = 0;
+
 
 // Per-matrix 64-row block table (built on first use).
 static int ensure_sweep_tab(ttad_matrix* M) {
@@ -279,7 +284,7 @@ int ttad_posv_mixed(ttad_context* ctx, ttad_matrix* A, int prec, int nrhs, const
     if (!W) return rc;
     if ((rc = ttad_matrix_copy(W, A, s))) return rc;
     int info = 0;
-    rc = ttad_potrf_tiled(ctx, W, TTAD_SCHED_PERSISTENT, 2, prec, s, &info);
+    rc = ttad_potrf_tiled(ctx, W, TTAD_SCHED_PERSISTENT, MIXED_PRIO_KIND, prec, s, &info);
     loc.factor_s = secs();
     loc.info = info;
     int iters = 0;
@@ -296,7 +301,7 @@ int ttad_posv_mixed(ttad_context* ctx, ttad_matrix* A, int prec, int nrhs, const
         ttad_matrix* W64 = working(ctx, A, P_F64, rc);
         if (!W64) return rc;
         if ((rc = ttad_matrix_copy(W64, A, s))) return rc;
-        rc = ttad_potrf_tiled(ctx, W64, TTAD_SCHED_PERSISTENT, 2, P_F64, s, &info);
+        rc = ttad_potrf_tiled(ctx, W64, TTAD_SCHED_PERSISTENT, MIXED_PRIO_KIND, P_F64, s, &info);
         loc.info = info;
         if (rc == TTAD_OK) {
             hipLaunchKernelGGL(k_zero_cols, dim3(256), dim3(256), 0, s, X, ldx, A->n, nrhs);

@@ -54,7 +54,7 @@ template <typename T> __device__ __forceinline__ T zero_val() { return T(0); }
 // loads of chunk c+2 are in flight (one barrier per chunk). A chunk's MFMA time
 // (BK=32 FP64: 8 k-steps x 16 MFMAs per wave) is what hides the load latency.
 // ---------------------------------------------------------------------------
-template <typename TA, typename TB, typename TC, int BM, int BN, int BK>
+template <typename TA, typename TB, typename TC, int BM, int BN, int BK, int DEPTH = 1>
 struct GemmNT {
     static constexpr int PA = BM + 2;
     static constexpr int PB = BN + 2;
@@ -129,9 +129,58 @@ struct GemmNT {
         }
     }
 
+    // DEPTH = 2: two register sets, so that a chunk's global loads are issued two chunks
+    // before its LDS store (twice the latency budget, twice the prefetch registers).
+    static __device__ __forceinline__ void run2(acc_t (&acc)[WM][WN], const TA* __restrict__ A, long lda, int mv,
+                                                const TB* __restrict__ B, long ldb, int nv, int K, char* lds) {
+        TC* S0 = reinterpret_cast<TC*>(lds);
+        TC* S1 = S0 + STAGE;
+        const int tid = threadIdx.x;
+        const int lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
+        const int ra_row = tid % BM, ra_k = tid / BM;
+        const int rb_row = tid % BN, rb_k = tid / BN;
+        const bool a_ok = ra_row < mv, b_ok = rb_row < nv;
+        const TA* Ab = A + ra_row;
+        const TB* Bb = B + rb_row;
+        const int nchunks = (K + BK - 1) / BK;
+        __syncthreads();
+        if (nchunks == 0) return;
+        TA ra0[EA], ra1[EA];
+        TB rb0[EB], rb1[EB];
+        auto load = [&](TA (&ra)[EA], TB (&rb)[EB], int c) {
+            load_chunk<TA, EA, KS_A>(ra, Ab, lda, a_ok, c * BK + ra_k, K);
+            load_chunk<TB, EB, KS_B>(rb, Bb, ldb, b_ok, c * BK + rb_k, K);
+        };
+        load(ra0, rb0, 0);
+        store_chunk(S0, ra0, rb0, ra_row, ra_k, rb_row, rb_k);
+        if (nchunks > 2) load(ra0, rb0, 2);
+        if (nchunks > 1) load(ra1, rb1, 1);
+        __syncthreads();
+        for (int c = 0; c < nchunks; c += 2) {
+            // even chunk c in S0; chunk c+1 waits in set 1, chunk c+2 in set 0
+            compute(acc, S0, lane, wr, wc);
+            if (c + 1 < nchunks) {
+                store_chunk(S1, ra1, rb1, ra_row, ra_k, rb_row, rb_k);
+                if (c + 3 < nchunks) load(ra1, rb1, c + 3);
+            }
+            __syncthreads();
+            if (c + 1 >= nchunks) break;
+            compute(acc, S1, lane, wr, wc);
+            if (c + 2 < nchunks) {
+                store_chunk(S0, ra0, rb0, ra_row, ra_k, rb_row, rb_k);
+                if (c + 4 < nchunks) load(ra0, rb0, c + 4);
+            }
+            __syncthreads();
+        }
+    }
+
     // Starts and ends with a workgroup barrier: the caller may use `lds` before and after.
     static __device__ __forceinline__ void run(acc_t (&acc)[WM][WN], const TA* __restrict__ A, long lda, int mv,
                                const TB* __restrict__ B, long ldb, int nv, int K, char* lds) {
+        if constexpr (DEPTH == 2) {
+            run2(acc, A, lda, mv, B, ldb, nv, K, lds);
+            return;
+        }
         TC* S0 = reinterpret_cast<TC*>(lds);
         const int tid = threadIdx.x;
         const int lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
@@ -332,9 +381,9 @@ struct GemmNTbf16 {
     }
 };
 
-// K-chunk of the 128x128 GEMM items: 32 for FP64 (8 k-steps of 64-cycle MFMAs per
-// chunk), 64 for FP32 (the MFMA is twice as fast); both fill 130 KB of LDS.
-template <typename TC> constexpr int gemm_bk() { return sizeof(TC) == 8 ? 32 : 64; }
+// K-chunk of the 128x128 FP64/FP32 GEMM items: 32 (FP64: 8 k-steps of 64-cycle MFMAs per
+// chunk, 130 KB of LDS; FP32: 65 KB, so that two workgroups fit a CU where registers allow).
+template <typename TC> constexpr int gemm_bk() { return 32; }
 
 // ---------------------------------------------------------------------------
 // GEMM / SYRK work item: C[r0:r0+128, c0:c0+128] = beta*C + alpha * A[r0:,:] B[c0:,:]^T
@@ -342,7 +391,7 @@ template <typename TC> constexpr int gemm_bk() { return sizeof(TC) == 8 ? 32 : 6
 // lower triangle (global row >= global col) is written (dsyrk 'L').
 // beta == 0 overwrites C without reading it (accumulate-mode temporaries).
 // ---------------------------------------------------------------------------
-template <typename TS, int PREC>
+template <typename TS, int PREC, int DEPTH = 1>
 __device__ __forceinline__ void item_gemm(const TS* __restrict__ A, long lda, const TS* __restrict__ B, long ldb, TS* __restrict__ C,
                           long ldc, int m, int n, int K, int r0, int c0, bool syrk, double alpha, double beta,
                           char* lds) {
@@ -368,7 +417,7 @@ __device__ __forceinline__ void item_gemm(const TS* __restrict__ A, long lda, co
                 }
     } else {
         typedef TS TC;  // F64 tiles on FP64 MFMA, F32 tiles on FP32 MFMA
-        using G = GemmNT<TS, TS, TC, GEMM_BM, GEMM_BM, gemm_bk<TC>()>;
+        using G = GemmNT<TS, TS, TC, GEMM_BM, GEMM_BM, gemm_bk<TC>(), DEPTH>;
         typename G::acc_t acc[G::WM][G::WN];
         G::zero(acc);
         G::run(acc, A + r0, lda, mv, B + c0, ldb, nv, K, lds);
@@ -680,6 +729,11 @@ __device__ __forceinline__ void item_trsm(const TS* __restrict__ L, long ldl, co
 constexpr int LDS_GEMM_F64 = GemmNT<double, double, double, GEMM_BM, GEMM_BM, gemm_bk<double>()>::LDS_BYTES;
 constexpr int LDS_GEMM_F32 = GemmNT<float, float, float, GEMM_BM, GEMM_BM, gemm_bk<float>()>::LDS_BYTES;
 constexpr int LDS_GEMM_BF16 = GemmNTbf16<float, GEMM_BM, GEMM_BM, true>::LDS_BYTES;
+constexpr int LDS_GEMM_BF16_1 = GemmNTbf16<float, GEMM_BM, GEMM_BM, false>::LDS_BYTES;
+// LDS of one GEMM item of the given storage / precision (the standalone kernels launch with it).
+template <typename TS, int PREC> constexpr int gemm_item_lds() {
+    return PREC == P_BF16X3 ? LDS_GEMM_BF16 : PREC == P_BF16 ? LDS_GEMM_BF16_1 : sizeof(TS) == 8 ? LDS_GEMM_F64 : LDS_GEMM_F32;
+}
 constexpr int LDS_POTRF = PotrfGemm<double>::G::LDS_BYTES > 2 * DB * DP * 8 ? PotrfGemm<double>::G::LDS_BYTES
                                                                              : 2 * DB * DP * 8;
 constexpr int LDS_TRSM = GemmNT<double, double, double, TRSM_BM, DB, 32>::LDS_BYTES;
