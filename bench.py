@@ -60,6 +60,20 @@ def make_matrix(n: int, seed_b: int, seed_d: int, delta: float, scaled: bool):
     return A  # symmetric, so row-major storage equals column-major
 
 
+def measured_traffic(n: int, b: int):
+    """HBM bytes per launch of the FP64 factorization kernel, as measured offline with
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by a 1 GiB copy (profiles/), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.txt")
+    try:
+        for line in open(path):
+            key, _, val = line.partition("=")
+            if key.strip() == f"cholesky_fp64_n{n}_b{b}_bytes":
+                return float(val)
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(n_sample: int, b: int, threads: int, prio: int = 0):
     """The oracle (Taskflow+Threadpool restatement, sequential MKL per task) on the host cores."""
     import numpy as np
@@ -182,6 +196,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson request rate of the serving leg (1/s)")
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run a 1 GiB device copy first (reference kernel for HBM byte counters)")
     ap.add_argument("--prio", type=int, default=0,
                     help="PrioKind of the FP64 leg and the CPU baseline (the mixed legs use 0, the miniapp default)")
     args = ap.parse_args()
@@ -211,6 +227,13 @@ def main():
     W64 = tt.TiledMatrix(ctx, sizes, "f64")
     ctx.synchronize()
     torch.cuda.synchronize()
+    if args.calib_copy:  # a plain copy of known size: reads 1 GiB, writes 1 GiB
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # ---- FP64 leg: factor + FP64 refinement ----
     def fp64_step():
@@ -305,7 +328,8 @@ def main():
                    "n": n, "tile": b, "nrhs": nrhs, "prio_kind": args.prio, "schedule": "persistent"},
         "legs": res,
         "roofline": {"bound": "mfma", "achieved": round(F / t_fac64 / 1e12, 3), "peak": PEAK["f64"],
-                     "unit": "TFLOP/s", "frac": round(F / t_fac64 / 1e12 / PEAK["f64"], 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(F / t_fac64 / 1e12 / PEAK["f64"], 4),
+                     "traffic": measured_traffic(n, b),
                      "kernel": "k_persistent FP64 factorization (one launch, F(n) flops / event-timed duration)"},
     }
     if not args.no_serve:

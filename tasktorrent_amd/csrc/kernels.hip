@@ -105,7 +105,9 @@ __global__ __launch_bounds__(NTHREADS) void k_persistent(DevGraph g) {
                         }
                         if (slot >= 0) {
                             const int64_t* sp = g.q_items + g.q_base[fb] + slot;
-                            while ((e = ld_acquire64(sp)) < 0) {  // claimed before its producer wrote it
+                            // claimed before its producer wrote it: poll relaxed (an acquire load would
+                            // invalidate the L2 on every poll); the acquire fence follows the broadcast
+                            while ((e = ld_relaxed64(sp)) < 0) {
                                 if ((int64_t)wall_clock64() - t_start > g.spin_limit) {
                                     __hip_atomic_store(g.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                                     e = -2;

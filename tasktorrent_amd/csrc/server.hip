@@ -44,7 +44,8 @@ __device__ __forceinline__ int64_t srv_try_pop(const DevServer& s, uint32_t& rng
         const unsigned long long* slot = R.slots + (h & (cap - 1));
         const int64_t t0 = (int64_t)wall_clock64();
         for (;;) {
-            const unsigned long long e = __hip_atomic_load(slot, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            // relaxed: an acquire load would invalidate the L2 on every poll; the caller fences once
+            const unsigned long long e = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if ((e >> 40) == lap) return (int64_t)(e & 0xFFFFFFFFFFull);
             if ((int64_t)wall_clock64() - t0 > s.spin_limit ||
                 __hip_atomic_load(s.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0) {
@@ -77,15 +78,17 @@ __global__ __launch_bounds__(NTHREADS) void k_server(DevServer s) {
                 }
                 // doorbell (host memory): polled by one workgroup in eight to limit bus traffic
                 const bool poller = (blockIdx.x & 7) == 0;
-                const int posted = poller ? __hip_atomic_load((const int32_t*)s.db_count, __ATOMIC_ACQUIRE,
+                // (relaxed polls: an acquire load invalidates caches every time; one fence once posted)
+                const int posted = poller ? __hip_atomic_load((const int32_t*)s.db_count, __ATOMIC_RELAXED,
                                                               __HIP_MEMORY_SCOPE_SYSTEM)
                                           : 0;
                 int taken = __hip_atomic_load(s.db_taken, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (poller && taken < posted) {
                     if (__hip_atomic_compare_exchange_strong(s.db_taken, &taken, taken + 1, __ATOMIC_RELAXED,
                                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        __atomic_thread_fence(__ATOMIC_ACQUIRE);  // system scope: the entry was written before the count
                         const int slot = __hip_atomic_load((const int32_t*)(s.db_entries + (taken % s.db_cap)),
-                                                           __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                         got = -100 - (int64_t)slot;  // admission, done by the whole workgroup
                         break;
                     }

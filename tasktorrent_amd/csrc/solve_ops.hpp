@@ -99,26 +99,35 @@ __device__ __forceinline__ void block_mma(typename Mfma<TC>::acc_t (&acc)[NR / 1
     }
 }
 
-// Waits for *f != 0. Gives up after `limit` ticks (sets *err = 1) or as soon as *err is
-// nonzero (an earlier wait timed out, or the owner asked the kernel to stop), so that a
-// failed run drains in one time limit rather than one per flag.
-__device__ __forceinline__ void spin_flag(const int32_t* f, int32_t* err, int64_t limit) {
+// Waits until *count >= want and returns the value seen. The polls are relaxed (an acquire
+// load invalidates the L2 on every iteration, which with hundreds of waiting workgroups
+// evicts everyone's data); one agent-scope acquire follows. Gives up after `limit` ticks
+// (sets *err = 1) or as soon as *err is nonzero (an earlier wait timed out, or the owner
+// asked the kernel to stop), so that a failed run drains in one time limit.
+__device__ __forceinline__ int wait_count(const int32_t* count, int want, int32_t* err, int64_t limit) {
     const int64_t t0 = (int64_t)wall_clock64();
-    while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0) return;
+    int v;
+    while ((v = __hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < want) {
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0) { v = want; break; }
         if ((int64_t)wall_clock64() - t0 > limit) {
             __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            return;
+            v = want;
+            break;
         }
         __builtin_amdgcn_s_sleep(1);
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return v;
 }
 
 // One 64-row block p of a triangular sweep with the factor held in M.
 // forward != 0: Y[p] = L[p,p]^{-1} (R[p] - sum_{q<p} L[p,q] Y[q])          (R FP64, Y in TC)
 // forward == 0: D[p] = L[p,p]^{-T} (Y[p] - sum_{q>p} L[q,p]^T D[q]), X[p] += D[p] (FP64)
-// Waits (acquire) on flags[q] of every block it reads and publishes flags[p] (release).
-// The caller guarantees that every block this one waits on belongs to a running workgroup.
+// Blocks finish in sweep order (block p needs every earlier block), so progress is one
+// counter, flags[0] = number of finished blocks: a block waits only when it reaches a
+// predecessor beyond the count it last saw, and one acquire then covers all of them.
+// The counter is incremented (release) when the block is done. The caller guarantees
+// that every block this one waits on belongs to a running workgroup.
 template <typename TL, typename TC, int NR>
 __device__ __forceinline__ void item_sweep(const DevMat& M, const SweepTab& tab, int forward, int p, const double* R,
                                            long ldr, TC* Y, long ldy, TC* D, long ldd, double* X, long ldx, int nrhs,
@@ -133,14 +142,21 @@ __device__ __forceinline__ void item_sweep(const DevMat& M, const SweepTab& tab,
     typename Mfma<TC>::acc_t acc[NB];
 #pragma unroll
     for (int nb = 0; nb < NB; nb++) acc[nb] = typename Mfma<TC>::acc_t{0, 0, 0, 0};
-    const int q_begin = forward ? 0 : P - 1, q_step = forward ? 1 : -1;
-    for (int q = q_begin; q != p; q += q_step) {
-        // the factor block does not depend on the flag: fetch it first
+    __shared__ int s_seen;
+    const int need = forward ? p : P - 1 - p;  // predecessors, in sweep order
+    int seen = 0;                               // predecessors known to be finished
+    for (int i = 0; i < need; i++) {
+        const int q = forward ? i : P - 1 - i;
+        // the factor block does not depend on the counter: fetch it first
         if (forward) fill_block<TL, TC>(Ls, M, tab, p, q, 0, false);
         else fill_block<TL, TC>(Ls, M, tab, p, q, 1, false);
-        if (threadIdx.x == 0) spin_flag(flags + q, err, spin_limit);
+        if (i >= seen) {  // uniform
+            if (threadIdx.x == 0) s_seen = wait_count(flags, i + 1, err, spin_limit);
+            __syncthreads();
+            seen = s_seen;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
         __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         const int gq = blk_g0(M, tab, q), wq = blk_rows(M, tab, q);
         if (forward) fill_rhs<TC, TC, NR>(Ys, Y, ldy, gq, wq, nrhs);
         else fill_rhs<TC, TC, NR>(Ys, D, ldd, gq, wq, nrhs);
@@ -187,7 +203,7 @@ __device__ __forceinline__ void item_sweep(const DevMat& M, const SweepTab& tab,
     __syncthreads();
     if (threadIdx.x == 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_store(flags + p, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(flags, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
