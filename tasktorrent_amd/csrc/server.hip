@@ -19,7 +19,7 @@ namespace ttad {
 // ---------------------------------------------------------------------------
 // Device loop
 // ---------------------------------------------------------------------------
-constexpr int SRV_LDS = cmax(LDS_ITEM_MAX, sweep_lds_bytes<double, 32>());
+constexpr int SRV_LDS = cmax(LDS_ITEM_MAX, sweep_lds_bytes<double, 32, false>());
 
 __device__ __forceinline__ int64_t srv_try_pop(const DevServer& s, uint32_t& rng) {
     // lane 0 only: claim the head of the first non-empty ring in priority order: a fetch-add

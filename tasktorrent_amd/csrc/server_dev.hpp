@@ -204,7 +204,7 @@ __device__ __forceinline__ void srv_exec(const DevServer& s, const SrvTask& t, i
         }
         case S_ANORM: {
             const SweepTab tab{q.P, q.blk_tile, q.blk_off};
-            item_resid<16>(s.mats[q.mat_a], tab, it, q.B, q.n, q.X, q.n, q.R, q.n, q.nrhs, q.anorm, 1, lds);
+            item_resid<16, false>(s.mats[q.mat_a], tab, it, q.B, q.n, q.X, q.n, q.R, q.n, q.nrhs, q.anorm, 1, lds);
             break;
         }
         case S_FWD:
@@ -216,17 +216,17 @@ __device__ __forceinline__ void srv_exec(const DevServer& s, const SrvTask& t, i
             int32_t* fl = fwd ? q.flags_f : q.flags_b;
             if (W.storage == P_F64) {
                 if (q.nrhs <= 16)
-                    item_sweep<double, double, 16>(W, tab, fwd, p, q.R, q.n, (double*)q.Y, q.n, (double*)q.D, q.n, q.X,
+                    item_sweep<double, double, 16, false>(W, tab, fwd, p, q.R, q.n, (double*)q.Y, q.n, (double*)q.D, q.n, q.X,
                                                    q.n, q.nrhs, fl, s.err, s.spin_limit, lds);
                 else
-                    item_sweep<double, double, 32>(W, tab, fwd, p, q.R, q.n, (double*)q.Y, q.n, (double*)q.D, q.n, q.X,
+                    item_sweep<double, double, 32, false>(W, tab, fwd, p, q.R, q.n, (double*)q.Y, q.n, (double*)q.D, q.n, q.X,
                                                    q.n, q.nrhs, fl, s.err, s.spin_limit, lds);
             } else {
                 if (q.nrhs <= 16)
-                    item_sweep<float, float, 16>(W, tab, fwd, p, q.R, q.n, (float*)q.Y, q.n, (float*)q.D, q.n, q.X,
+                    item_sweep<float, float, 16, false>(W, tab, fwd, p, q.R, q.n, (float*)q.Y, q.n, (float*)q.D, q.n, q.X,
                                                  q.n, q.nrhs, fl, s.err, s.spin_limit, lds);
                 else
-                    item_sweep<float, float, 32>(W, tab, fwd, p, q.R, q.n, (float*)q.Y, q.n, (float*)q.D, q.n, q.X,
+                    item_sweep<float, float, 32, false>(W, tab, fwd, p, q.R, q.n, (float*)q.Y, q.n, (float*)q.D, q.n, q.X,
                                                  q.n, q.nrhs, fl, s.err, s.spin_limit, lds);
             }
             break;
@@ -234,9 +234,9 @@ __device__ __forceinline__ void srv_exec(const DevServer& s, const SrvTask& t, i
         case S_RESID: {
             const SweepTab tab{q.P, q.blk_tile, q.blk_off};
             if (q.nrhs <= 16)
-                item_resid<16>(s.mats[q.mat_a], tab, it, q.B, q.n, q.X, q.n, q.R, q.n, q.nrhs, q.norms, 0, lds);
+                item_resid<16, false>(s.mats[q.mat_a], tab, it, q.B, q.n, q.X, q.n, q.R, q.n, q.nrhs, q.norms, 0, lds);
             else
-                item_resid<32>(s.mats[q.mat_a], tab, it, q.B, q.n, q.X, q.n, q.R, q.n, q.nrhs, q.norms, 0, lds);
+                item_resid<32, false>(s.mats[q.mat_a], tab, it, q.B, q.n, q.X, q.n, q.R, q.n, q.nrhs, q.norms, 0, lds);
             break;
         }
         case S_CHECK: {

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "runtime.hpp"
@@ -35,8 +36,17 @@ static int ensure_sweep_tab(ttad_matrix* M) {
     return TTAD_OK;
 }
 
+// Sweep waiting policy; TTAD_SWEEP_NEAR / TTAD_SWEEP_BATCH override the defaults (tuning).
+static int env_int(const char* name, int dflt, int lo, int hi) {
+    const char* v = getenv(name);
+    if (!v || !*v) return dflt;
+    return std::min(hi, std::max(lo, atoi(v)));
+}
+
 static SweepTab tab_of(const ttad_matrix* M) {
-    return SweepTab{M->sweep_P, M->tab_blk_tile.as<int32_t>(), M->tab_blk_off.as<int32_t>()};
+    static const int near = env_int("TTAD_SWEEP_NEAR", SWEEP_NEAR, 1, 1 << 20);
+    static const int batch = env_int("TTAD_SWEEP_BATCH", SWEEP_BATCH, 1, 1 << 20);
+    return SweepTab{M->sweep_P, M->tab_blk_tile.as<int32_t>(), M->tab_blk_off.as<int32_t>(), near, batch};
 }
 
 static int ensure(DevBuf& b, size_t bytes) {

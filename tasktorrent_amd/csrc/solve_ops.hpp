@@ -11,7 +11,7 @@
 //
 // Sweeps run one workgroup per block. Blocks are claimed in dependency order
 // through a ticket counter (so every block a workgroup waits on is owned by a
-// workgroup that is already running), and published with a release flag.
+// workgroup that is already running), and published through a release counter.
 #pragma once
 #include "tile_ops.hpp"
 
@@ -19,11 +19,24 @@ namespace ttad {
 
 constexpr int SB = 64;        // rows per sweep block
 constexpr int SPAD = 4;       // LDS row padding (elements)
+constexpr int KS = SB / 4;    // MFMA k-steps of one 64-wide block product
+
+// Sweep waiting policy: a block with at most `near` predecessors left waits for each
+// one as it finishes; a block further from the front waits until `batch` more (up to
+// `near` short of its own turn) are done, then takes them in one go with one acquire,
+// and polls slowly meanwhile. Measured at n = 32768 (512 blocks, FP64, 16 rhs): batches
+// make the far blocks fall behind the front (batch 8: 13.4 ms for both sweeps, batch 32:
+// 31 ms), so the default takes every predecessor as soon as it is final (8.1 ms), with
+// only the slow polling of the blocks that are not next in line.
+constexpr int SWEEP_NEAR = 1;
+constexpr int SWEEP_BATCH = 1;
 
 struct SweepTab {
     int32_t P;                // number of 64-row blocks
     const int32_t* blk_tile;  // [P]
     const int32_t* blk_off;   // [P]
+    int32_t near = SWEEP_NEAR;
+    int32_t batch = SWEEP_BATCH;
 };
 
 __device__ __forceinline__ int blk_rows(const DevMat& M, const SweepTab& t, int p) {
@@ -31,6 +44,75 @@ __device__ __forceinline__ int blk_rows(const DevMat& M, const SweepTab& t, int 
 }
 __device__ __forceinline__ int blk_g0(const DevMat& M, const SweepTab& t, int p) {
     return M.boff[t.blk_tile[p]] + t.blk_off[p];
+}
+
+// Block products run on 16x16x4 MFMAs with operands read straight from memory into
+// registers (no LDS, no barrier): wave w owns rows 16w..16w+15 of the 64-row block, and
+// in k-step ks lane l supplies A[m = 16w + l%16][k = 4ks + l/16] and B[k][n = 16nb + l%16].
+
+// A operand of wave w: rows of block p x columns of block q of the symmetric/triangular
+// matrix held in the lower tiles of M.
+//   mode 0: plain block L[p,q] (requires tile(p) >= tile(q))
+//   mode 1: transposed block L[q,p]^T (requires tile(q) >= tile(p))
+//   mode 2: symmetric diagonal block (p == q, lower part stored)
+// Out-of-range rows/cols read as 0.
+template <typename TL>
+__device__ __forceinline__ void load_a(TL (&a)[KS], const DevMat& M, const SweepTab& tab, int p, int q, int mode) {
+    const int tp = tab.blk_tile[p], tq = tab.blk_tile[q];
+    const int op = tab.blk_off[p], oq = tab.blk_off[q];
+    const int wp = min(SB, M.bsz[tp] - op), wq = min(SB, M.bsz[tq] - oq);
+    const int lane = threadIdx.x & 63;
+    const int m = (threadIdx.x >> 6) * 16 + (lane & 15), kg = lane >> 4;
+    const bool mok = m < wp;
+    if (mode == 0) {
+        const long ld = M.bsz[tp];
+        const TL* T = (const TL*)M.tiles[tp + (long)tq * M.nt] + (op + m) + (long)oq * ld;
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const int k = 4 * ks + kg;
+            a[ks] = (mok && k < wq) ? T[(long)k * ld] : TL(0);
+        }
+    } else if (mode == 1) {
+        const long ld = M.bsz[tq];
+        const TL* T = (const TL*)M.tiles[tq + (long)tp * M.nt] + oq + (long)(op + m) * ld;
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const int k = 4 * ks + kg;
+            a[ks] = (mok && k < wq) ? T[k] : TL(0);
+        }
+    } else {
+        const long ld = M.bsz[tp];
+        const TL* T = (const TL*)M.tiles[tp + (long)tp * M.nt] + op + (long)op * ld;
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const int k = 4 * ks + kg;
+            a[ks] = (mok && k < wp) ? (m >= k ? T[m + (long)k * ld] : T[k + (long)m * ld]) : TL(0);
+        }
+    }
+}
+
+// B operand: rows [g0, g0 + w) of a column-major n x nrhs array (k = row within the block).
+template <typename TS, typename TC, int NR>
+__device__ __forceinline__ void load_b(TC (&b)[NR / 16][KS], const TS* Y, long ldy, int g0, int w, int nrhs) {
+    const int lane = threadIdx.x & 63, kg = lane >> 4;
+#pragma unroll
+    for (int nb = 0; nb < NR / 16; nb++) {
+        const int n = nb * 16 + (lane & 15);
+        const TS* Yc = Y + g0 + (long)n * ldy;
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const int k = 4 * ks + kg;
+            b[nb][ks] = (k < w && n < nrhs) ? (TC)Yc[k] : TC(0);
+        }
+    }
+}
+
+template <typename TC, typename TL, int NB>
+__device__ __forceinline__ void reg_mma(typename Mfma<TC>::acc_t (&acc)[NB], const TL (&a)[KS], const TC (&b)[NB][KS]) {
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) acc[nb] = Mfma<TC>::mma((TC)a[ks], b[nb][ks], acc[nb]);
 }
 
 // Fill Ls[k][m] (TC, k-major, ld 64+SPAD) with the 64x64 block (rows of block p, cols of block q)
@@ -103,8 +185,10 @@ __device__ __forceinline__ void block_mma(typename Mfma<TC>::acc_t (&acc)[NR / 1
 // load invalidates the L2 on every iteration, which with hundreds of waiting workgroups
 // evicts everyone's data); one agent-scope acquire follows. Gives up after `limit` ticks
 // (sets *err = 1) or as soon as *err is nonzero (an earlier wait timed out, or the owner
-// asked the kernel to stop), so that a failed run drains in one time limit.
-__device__ __forceinline__ int wait_count(const int32_t* count, int want, int32_t* err, int64_t limit) {
+// asked the kernel to stop), so that a failed run drains in one time limit. `slow` polls
+// about once a microsecond instead of back to back.
+__device__ __forceinline__ int wait_count(const int32_t* count, int want, int32_t* err, int64_t limit,
+                                          bool slow = false) {
     const int64_t t0 = (int64_t)wall_clock64();
     int v;
     while ((v = __hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < want) {
@@ -114,7 +198,8 @@ __device__ __forceinline__ int wait_count(const int32_t* count, int want, int32_
             v = want;
             break;
         }
-        __builtin_amdgcn_s_sleep(1);
+        if (slow) __builtin_amdgcn_s_sleep(32);
+        else __builtin_amdgcn_s_sleep(1);
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     return v;
@@ -125,69 +210,125 @@ __device__ __forceinline__ int wait_count(const int32_t* count, int want, int32_
 // forward == 0: D[p] = L[p,p]^{-T} (Y[p] - sum_{q>p} L[q,p]^T D[q]), X[p] += D[p] (FP64)
 // Blocks finish in sweep order (block p needs every earlier block), so progress is one
 // counter, flags[0] = number of finished blocks: a block waits only when it reaches a
-// predecessor beyond the count it last saw, and one acquire then covers all of them.
-// The counter is incremented (release) when the block is done. The caller guarantees
-// that every block this one waits on belongs to a running workgroup.
-template <typename TL, typename TC, int NR>
+// predecessor beyond the count it last saw (policy in SweepTab), and one acquire by the
+// polling thread, passed on by the barrier, then covers all of them. The next factor
+// block, and the next right-hand side block once it is known to be final, are loaded
+// while the current product runs. The counter is incremented (release) when the block
+// is done. The caller guarantees that every block this one waits on belongs to a
+// running workgroup.
+template <typename TL, typename TC, int NR, bool PF = true>
 __device__ __forceinline__ void item_sweep(const DevMat& M, const SweepTab& tab, int forward, int p, const double* R,
                                            long ldr, TC* Y, long ldy, TC* D, long ldd, double* X, long ldx, int nrhs,
                                            int32_t* flags, int32_t* err, int64_t spin_limit, char* lds) {
     constexpr int NB = NR / 16;
     constexpr int LD = SB + SPAD, LDY = NR + SPAD;
-    TC* Ls = reinterpret_cast<TC*>(lds);
-    TC* Ys = Ls + SB * LD;
+    TC* Ds = reinterpret_cast<TC*>(lds);  // diagonal inverse, Ds[k][m] = A operand (m, k)
+    TC* Ys = Ds + SB * LD;                // t = rhs - sum, B operand of the diagonal product
     const int P = tab.P;
     const int g0 = blk_g0(M, tab, p), w = blk_rows(M, tab, p);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    typename Mfma<TC>::acc_t acc[NB];
+    const TC* Src = forward ? Y : D;  // right-hand side blocks of the predecessors
+    const long lds_ = forward ? ldy : ldd;
+    typedef typename Mfma<TC>::acc_t acc_t;
+    acc_t acc[NB];
 #pragma unroll
-    for (int nb = 0; nb < NB; nb++) acc[nb] = typename Mfma<TC>::acc_t{0, 0, 0, 0};
-    __shared__ int s_seen;
-    const int need = forward ? p : P - 1 - p;  // predecessors, in sweep order
-    int seen = 0;                               // predecessors known to be finished
-    for (int i = 0; i < need; i++) {
-        const int q = forward ? i : P - 1 - i;
-        // the factor block does not depend on the counter: fetch it first
-        if (forward) fill_block<TL, TC>(Ls, M, tab, p, q, 0, false);
-        else fill_block<TL, TC>(Ls, M, tab, p, q, 1, false);
-        if (i >= seen) {  // uniform
-            if (threadIdx.x == 0) s_seen = wait_count(flags, i + 1, err, spin_limit);
-            __syncthreads();
-            seen = s_seen;
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        }
-        __syncthreads();
-        const int gq = blk_g0(M, tab, q), wq = blk_rows(M, tab, q);
-        if (forward) fill_rhs<TC, TC, NR>(Ys, Y, ldy, gq, wq, nrhs);
-        else fill_rhs<TC, TC, NR>(Ys, D, ldd, gq, wq, nrhs);
-        __syncthreads();
-        block_mma<TC, NR>(acc, Ls, Ys);
-        __syncthreads();
-    }
-    // t = rhs block - acc, staged in Ys as the B operand of the diagonal product
+    for (int nb = 0; nb < NB; nb++) acc[nb] = acc_t{0, 0, 0, 0};
+
+    // Off the critical path: this block's right-hand side and diagonal inverse.
+    TC rhs[NB][4];
+#pragma unroll
     for (int nb = 0; nb < NB; nb++)
+#pragma unroll
         for (int r = 0; r < 4; r++) {
             const int m = wv * 16 + Mfma<TC>::row(lane, r), n = nb * 16 + (lane & 15);
-            TC v = TC(0);
-            if (m < w && n < nrhs) {
-                const TC rhs = forward ? (TC)R[(g0 + m) + (long)n * ldr] : Y[(g0 + m) + (long)n * ldy];
-                v = rhs - acc[nb][r];
-            }
-            Ys[m * LDY + n] = v;
+            rhs[nb][r] = TC(0);
+            if (m < w && n < nrhs)
+                rhs[nb][r] = forward ? (TC)R[(g0 + m) + (long)n * ldr] : Y[(g0 + m) + (long)n * ldy];
         }
-    // Dinv of the 64x64 diagonal block: forward uses it, backward its transpose
-    const int tp = tab.blk_tile[p];
-    const double* Dv = M.dinv[tp] + (long)(tab.blk_off[p] / SB) * SB * SB;
-    for (int e = threadIdx.x; e < SB * SB; e += NTHREADS) {
-        const int a = e % SB, b = e / SB;  // Dv[a + b*64] = Dinv(a, b)
-        if (forward) Ls[b * LD + a] = (TC)Dv[e];   // Ls[k][m] = Dinv(m, k)
-        else Ls[a * LD + b] = (TC)Dv[e];           // Ls[k][m] = Dinv(k, m) = Dinv^T(m, k)
+    {
+        const double* Dv = M.dinv[tab.blk_tile[p]] + (long)(tab.blk_off[p] / SB) * SB * SB;
+        for (int e = threadIdx.x; e < SB * SB; e += NTHREADS) {
+            const int a = e % SB, b = e / SB;  // Dv[a + b*64] = Dinv(a, b)
+            if (forward) Ds[b * LD + a] = (TC)Dv[e];   // (m, k) = Dinv(m, k)
+            else Ds[a * LD + b] = (TC)Dv[e];           // (m, k) = Dinv(k, m)
+        }
     }
+
+    // The count is passed through LDS slot (wait number & 1): a wave cannot be two
+    // barriers behind the polling thread, so a slot is never rewritten before it is read.
+    __shared__ int s_seen[2];
+    int nwait = 0;
+    const int need = forward ? p : P - 1 - p;  // predecessors, in sweep order
+    int seen = 0;                               // predecessors known to be finished
+    TL an[KS];                                  // factor block of the next predecessor
+    TC bn[NB][KS];                              // its right-hand side block, when prefetched
+    bool have_b = false;
+    if (PF && need > 0) load_a<TL>(an, M, tab, p, forward ? 0 : P - 1, forward ? 0 : 1);
+    for (int i = 0; i < need; i++) {
+        const int q = forward ? i : P - 1 - i;
+        if (i >= seen) {  // uniform
+            const int left = need - i;
+            const bool far = left > tab.near;
+            const int want = far ? min(i + tab.batch, need - tab.near) : i + 1;
+            if (threadIdx.x == 0) s_seen[nwait & 1] = wait_count(flags, want, err, spin_limit, far);
+            __syncthreads();
+            seen = s_seen[nwait & 1];
+            nwait++;
+        }
+        TC bc[NB][KS];
+        if constexpr (!PF) {  // LDS-staged operands: few live registers (kernels with other paths)
+            TC* Ls = Ys + SB * LDY;
+            TC* Bs = Ls + SB * LD;
+            fill_block<TL, TC>(Ls, M, tab, p, q, forward ? 0 : 1, false);
+            fill_rhs<TC, TC, NR>(Bs, Src, lds_, blk_g0(M, tab, q), blk_rows(M, tab, q), nrhs);
+            __syncthreads();
+            block_mma<TC, NR>(acc, Ls, Bs);
+            __syncthreads();
+            continue;
+        }
+        if (have_b) {
+#pragma unroll
+            for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+                for (int ks = 0; ks < KS; ks++) bc[nb][ks] = bn[nb][ks];
+        } else {
+            load_b<TC, TC, NR>(bc, Src, lds_, blk_g0(M, tab, q), blk_rows(M, tab, q), nrhs);
+        }
+        TL ac[KS];
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) ac[ks] = an[ks];
+        have_b = false;
+        if (i + 1 < need) {
+            const int qn = forward ? i + 1 : P - 2 - i;
+            load_a<TL>(an, M, tab, p, qn, forward ? 0 : 1);
+            if (i + 1 < seen) {
+                load_b<TC, TC, NR>(bn, Src, lds_, blk_g0(M, tab, qn), blk_rows(M, tab, qn), nrhs);
+                have_b = true;
+            }
+        }
+        reg_mma<TC, TL, NB>(acc, ac, bc);
+    }
+    // t = rhs block - acc, staged in Ys as the B operand of the diagonal product
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int m = wv * 16 + Mfma<TC>::row(lane, r), n = nb * 16 + (lane & 15);
+            Ys[m * LDY + n] = rhs[nb][r] - acc[nb][r];
+        }
     __syncthreads();
 #pragma unroll
-    for (int nb = 0; nb < NB; nb++) acc[nb] = typename Mfma<TC>::acc_t{0, 0, 0, 0};
-    block_mma<TC, NR>(acc, Ls, Ys);
+    for (int nb = 0; nb < NB; nb++) acc[nb] = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+        const int kr = ks * 4 + (lane >> 4);
+        const TC a = Ds[kr * LD + wv * 16 + (lane & 15)];
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) acc[nb] = Mfma<TC>::mma(a, Ys[kr * LDY + nb * 16 + (lane & 15)], acc[nb]);
+    }
+#pragma unroll
     for (int nb = 0; nb < NB; nb++)
+#pragma unroll
         for (int r = 0; r < 4; r++) {
             const int m = wv * 16 + Mfma<TC>::row(lane, r), n = nb * 16 + (lane & 15);
             if (m < w && n < nrhs) {
@@ -201,15 +342,13 @@ __device__ __forceinline__ void item_sweep(const DevMat& M, const SweepTab& tab,
             }
         }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_fetch_add(flags, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(flags, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <typename TC, int NR>
+// LDS of item_sweep<.., PF> (and of item_resid<NR, false> for TC = double).
+template <typename TC, int NR, bool PF = true>
 constexpr int sweep_lds_bytes() {
-    return SB * (SB + SPAD) * (int)sizeof(TC) + SB * (NR + SPAD) * (int)sizeof(TC);
+    return (PF ? 1 : 2) * (SB * (SB + SPAD) * (int)sizeof(TC) + SB * (NR + SPAD) * (int)sizeof(TC));
 }
 
 // Stand-alone sweep launch: one workgroup per block, blocks claimed through a ticket
@@ -232,40 +371,71 @@ __global__ __launch_bounds__(NTHREADS) void k_sweep(const DevMat* Mp, SweepTab t
 // Residual r = b - A x (FP64, A symmetric from its lower tiles) for one 64-row block p,
 // with per-rhs maxima of |r|, |x|, |b| (bit patterns of non-negative doubles are ordered
 // like the values, so an integer atomicMax reduces them). rownorm != 0 computes the row
-// sums of |A| instead (x ignored) and reduces their maximum into norms[0].
-template <int NR>
+// sums of |A| instead (x ignored) and reduces their maximum into norms[0]. The blocks of
+// the next column block are loaded while the current product runs.
+template <int NR, bool PF = true>
 __device__ __forceinline__ void item_resid(const DevMat& M, const SweepTab& tab, int p, const double* B, long ldb,
                                            const double* X, long ldx, double* R, long ldr, int nrhs,
                                            unsigned long long* norms, int rownorm, char* lds) {
     constexpr int NB = NR / 16;
-    constexpr int LD = SB + SPAD, LDY = NR + SPAD;
-    double* Ls = reinterpret_cast<double*>(lds);
-    double* Ys = Ls + SB * LD;
     const int g0 = blk_g0(M, tab, p), w = blk_rows(M, tab, p);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int tp = tab.blk_tile[p];
+    auto mode_of = [&](int q) {
+        const int tq = tab.blk_tile[q];
+        if (q == p) return 2;
+        if (tp > tq || (tp == tq && tab.blk_off[p] > tab.blk_off[q])) return 0;
+        return 1;
+    };
     d4_t acc[NB];
 #pragma unroll
     for (int nb = 0; nb < NB; nb++) acc[nb] = d4_t{0, 0, 0, 0};
-    for (int q = 0; q < tab.P; q++) {
-        const int tq = tab.blk_tile[q];
-        int mode;
-        if (q == p) mode = 2;
-        else if (tp > tq || (tp == tq && tab.blk_off[p] > tab.blk_off[q])) mode = 0;
-        else mode = 1;
-        fill_block<double, double>(Ls, M, tab, p, q, mode, rownorm != 0);
-        const int gq = blk_g0(M, tab, q), wq = blk_rows(M, tab, q);
+    double an[KS], bn[NB][KS];
+    auto load_next = [&](int q) {
+        load_a<double>(an, M, tab, p, q, mode_of(q));
         if (rownorm) {
-            for (int e = threadIdx.x; e < SB * NR; e += NTHREADS) {
-                const int k = e % SB, n = e / SB;
-                Ys[k * LDY + n] = (k < wq && n == 0) ? 1.0 : 0.0;
-            }
+            const int wq = blk_rows(M, tab, q), kg = lane >> 4;
+#pragma unroll
+            for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+                for (int ks = 0; ks < KS; ks++)
+                    bn[nb][ks] = (nb == 0 && (lane & 15) == 0 && 4 * ks + kg < wq) ? 1.0 : 0.0;
         } else {
-            fill_rhs<double, double, NR>(Ys, X, ldx, gq, wq, nrhs);
+            load_b<double, double, NR>(bn, X, ldx, blk_g0(M, tab, q), blk_rows(M, tab, q), nrhs);
         }
-        __syncthreads();
-        block_mma<double, NR>(acc, Ls, Ys);
-        __syncthreads();
+    };
+    if constexpr (!PF) {  // LDS-staged operands: few live registers (kernels with other paths)
+        constexpr int LDY = NR + SPAD;
+        double* Ls = reinterpret_cast<double*>(lds);
+        double* Ys = Ls + SB * (SB + SPAD);
+        for (int q = 0; q < tab.P; q++) {
+            fill_block<double, double>(Ls, M, tab, p, q, mode_of(q), rownorm != 0);
+            const int gq = blk_g0(M, tab, q), wq = blk_rows(M, tab, q);
+            if (rownorm) {
+                for (int e = threadIdx.x; e < SB * NR; e += NTHREADS) {
+                    const int k = e % SB, n = e / SB;
+                    Ys[k * LDY + n] = (k < wq && n == 0) ? 1.0 : 0.0;
+                }
+            } else {
+                fill_rhs<double, double, NR>(Ys, X, ldx, gq, wq, nrhs);
+            }
+            __syncthreads();
+            block_mma<double, NR>(acc, Ls, Ys);
+            __syncthreads();
+        }
+    } else {
+    load_next(0);
+    for (int q = 0; q < tab.P; q++) {
+        double ac[KS], bc[NB][KS];
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            ac[ks] = rownorm ? fabs(an[ks]) : an[ks];
+#pragma unroll
+            for (int nb = 0; nb < NB; nb++) bc[nb][ks] = bn[nb][ks];
+        }
+        if (q + 1 < tab.P) load_next(q + 1);
+        reg_mma<double, double, NB>(acc, ac, bc);
+    }
     }
     for (int nb = 0; nb < NB; nb++)
         for (int r = 0; r < 4; r++) {
@@ -289,9 +459,8 @@ template <int NR>
 __global__ __launch_bounds__(NTHREADS) void k_residual(const DevMat* Mp, SweepTab tab, const double* B, long ldb,
                                                        const double* X, long ldx, double* R, long ldr, int nrhs,
                                                        unsigned long long* norms, int rownorm) {
-    __shared__ __align__(16) char lds[sweep_lds_bytes<double, NR>()];
     if ((int)blockIdx.x >= tab.P) return;
-    item_resid<NR>(*Mp, tab, blockIdx.x, B, ldb, X, ldx, R, ldr, nrhs, norms, rownorm, lds);
+    item_resid<NR>(*Mp, tab, blockIdx.x, B, ldb, X, ldx, R, ldr, nrhs, norms, rownorm, nullptr);
 }
 
 }  // namespace ttad
