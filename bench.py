@@ -198,8 +198,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device copy first (reference kernel for HBM byte counters)")
-    ap.add_argument("--prio", type=int, default=0,
-                    help="PrioKind of the FP64 leg and the CPU baseline (the mixed legs use 0, the miniapp default)")
+    ap.add_argument("--prio", type=int, default=2,
+                    help="PrioKind of the FP64 leg and the CPU baseline (the mixed legs use 2 as well: critical path first)")
     args = ap.parse_args()
 
     if args.gpus != 1:

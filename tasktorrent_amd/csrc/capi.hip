@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <numeric>
@@ -124,7 +125,8 @@ int ttad_init(int device, ttad_context** out) {
         delete c;
         return set_error(TTAD_ERR_DEVICE, "init: %s", hipGetErrorString(e));
     }
-    c->workers_per_cu = persistent_blocks_per_cu();
+    for (int q = 0; q < N_VSETS; q++) c->vset_workers_per_cu[q] = persistent_blocks_per_cu(q);
+    c->workers_per_cu = c->vset_workers_per_cu[0];
     if (hipEventCreate(&c->ev_a) != hipSuccess || hipEventCreate(&c->ev_b) != hipSuccess) {
         delete c;
         return set_error(TTAD_ERR_DEVICE, "init: event creation failed");
@@ -493,6 +495,16 @@ int ttad_last_elapsed(ttad_context* ctx, double* seconds) {
 // ---------------------------------------------------------------------------
 // Device task graphs
 // ---------------------------------------------------------------------------
+// GEMM/SYRK work queues homed per XCD (default); TTAD_XCD_AFFINITY=0 turns them into one
+// shared queue per priority bucket (for comparisons).
+static bool xcd_affinity() {
+    static const bool on = [] {
+        const char* v = getenv("TTAD_XCD_AFFINITY");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+
 int ttad_graph_create(ttad_context* ctx, int n_tasks, const ttad_task_desc* tasks, const int32_t* indegree,
                       const int32_t* out_ptr, const int32_t* out_idx, const double* priority, ttad_graph** out) {
     if (!ctx || !out || n_tasks <= 0 || !tasks || !indegree || !out_ptr)
@@ -536,11 +548,29 @@ int ttad_graph_create(ttad_context* ctx, int n_tasks, const ttad_task_desc* task
     uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
     const int nu = (int)uniq.size();
     const int cap = 64;
-    g->n_buckets = std::min(nu, cap);
+    const int n_prio = std::min(nu, cap);
+    std::vector<int> pb(n_tasks), home(n_tasks, -1);
     for (int t = 0; t < n_tasks; t++) {
         const int rank = (int)(std::lower_bound(uniq.begin(), uniq.end(), pr[t], std::greater<double>()) - uniq.begin());
-        g->tasks[t].bucket = (nu <= cap) ? rank : (int)((long long)rank * cap / nu);
+        pb[t] = (nu <= cap) ? rank : (int)((long long)rank * cap / nu);
+        const DevTask& x = g->tasks[t];
+        // GEMM/SYRK items are homed by output tile row: the items of one update share their
+        // A and B panels, and the updates of one row share A, in one XCD's L2
+        if (xcd_affinity() && (x.kind == K_GEMM || x.kind == K_SYRK)) home[t] = x.i % NXCD;
     }
+    // work queues in priority order; within a bucket the shared queue, then one per home XCD
+    std::vector<int> qid((size_t)n_prio * (NXCD + 1), -1);
+    for (int t = 0; t < n_tasks; t++) qid[(size_t)pb[t] * (NXCD + 1) + (home[t] + 1)] = 0;
+    g->q_home.clear();
+    for (int b = 0; b < n_prio; b++)
+        for (int h = -1; h < NXCD; h++) {
+            int& q = qid[(size_t)b * (NXCD + 1) + (h + 1)];
+            if (q < 0) continue;
+            q = (int)g->q_home.size();
+            g->q_home.push_back(h);
+        }
+    g->n_buckets = (int)g->q_home.size();
+    for (int t = 0; t < n_tasks; t++) g->tasks[t].bucket = qid[(size_t)pb[t] * (NXCD + 1) + (home[t] + 1)];
     g->seeds.assign(n_tasks, 0);
     TTAD_HIP(hipEventCreate(&g->ev_start));
     TTAD_HIP(hipEventCreate(&g->ev_stop));
@@ -612,6 +642,7 @@ static int graph_prepare(ttad_graph* g) {
         TTAD_HIP(g->d_q_head.alloc(g->n_buckets * 4));
         TTAD_HIP(g->d_q_tail.alloc(g->n_buckets * 4));
         TTAD_HIP(g->d_q_base.alloc((g->n_buckets + 1) * 4));
+        TTAD_HIP(g->d_q_home.alloc(g->n_buckets * 4));
         TTAD_HIP(g->d_q_items.alloc((size_t)g->total_items * 8));
         TTAD_HIP(g->d_done.alloc(4));
         TTAD_HIP(g->d_abort.alloc(4));
@@ -622,6 +653,7 @@ static int graph_prepare(ttad_graph* g) {
         if (!g->out_idx.empty())
             TTAD_HIP(hipMemcpy(g->d_out_idx.p, g->out_idx.data(), g->out_idx.size() * 4, hipMemcpyHostToDevice));
         TTAD_HIP(hipMemcpy(g->d_q_base.p, g->q_base.data(), (g->n_buckets + 1) * 4, hipMemcpyHostToDevice));
+        TTAD_HIP(hipMemcpy(g->d_q_home.p, g->q_home.data(), g->n_buckets * 4, hipMemcpyHostToDevice));
         TTAD_HIP(hipMemcpy(g->d_mats.p, dm.data(), dm.size() * sizeof(DevMat), hipMemcpyHostToDevice));
         g->mats_dirty = false;
         g->seeds_dirty = true;
@@ -674,6 +706,7 @@ static DevGraph graph_image(ttad_graph* g, bool trace, int64_t spin_limit) {
     d.q_head = g->d_q_head.as<int32_t>();
     d.q_tail = g->d_q_tail.as<int32_t>();
     d.q_base = g->d_q_base.as<int32_t>();
+    d.q_home = g->d_q_home.as<int32_t>();
     d.q_items = g->d_q_items.as<int64_t>();
     d.done_items = g->d_done.as<int32_t>();
     d.abort_flag = g->d_abort.as<int32_t>();
@@ -696,7 +729,13 @@ int ttad_graph_launch(ttad_graph* g, int n_workers, int trace, void* stream) {
     (void)hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device);
     const int64_t spin_limit = (int64_t)rate_khz * 1000 * 5;  // 5 s without finding work
     const DevGraph d = graph_image(g, trace != 0, spin_limit);
-    g->workers = n_workers > 0 ? n_workers : ctx->n_cu * ctx->workers_per_cu;
+    static const int env_workers = [] {  // TTAD_WORKERS: default worker count override (tuning)
+        const char* v = getenv("TTAD_WORKERS");
+        return v ? atoi(v) : 0;
+    }();
+    const int dflt = env_workers > 0 ? std::min(env_workers, ctx->n_cu * ctx->vset_workers_per_cu[g->vset])
+                                     : ctx->n_cu * ctx->vset_workers_per_cu[g->vset];
+    g->workers = n_workers > 0 ? n_workers : dflt;
     g->stream = pick(ctx, stream);
     launch_graph_reset(d, g->total_items, g->stream);
     for (auto* M : g->mats)

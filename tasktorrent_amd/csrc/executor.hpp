@@ -24,6 +24,7 @@ struct DevGraph {
     int32_t* q_head;            // [n_buckets]
     int32_t* q_tail;            // [n_buckets]
     const int32_t* q_base;      // [n_buckets+1]
+    const int32_t* q_home;      // [n_buckets] -1 = any worker, else the XCD served first
     int64_t* q_items;           // entries (task << 20) | item; -1 = not yet written
     int32_t* done_items;
     int32_t* abort_flag;        // 1 = watchdog, 2 = queue overflow
@@ -32,6 +33,15 @@ struct DevGraph {
     int32_t* trace_cursor;
     int64_t spin_limit;         // watchdog in wall_clock64 ticks
 };
+
+// XCDs of the device (MI355X: 8 XCDs of 32 CUs, each with its own L2).
+constexpr int NXCD = 8;
+// Hardware id of the XCD (XCC) this wave runs on.
+__device__ __forceinline__ int xcc_id() {
+    unsigned v;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+    return (int)(v & 0xF) % NXCD;
+}
 
 __device__ __forceinline__ int ld_relaxed(const int32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -63,6 +73,19 @@ __host__ __device__ inline int item_variant(int kind, int prec, int storage) {
         return prec == P_F64 ? V_F64S : prec == P_F32 ? V_G32 : prec == P_BF16 ? V_GBF16 : V_GBF16X3;
     return storage == P_F64 ? V_F64S : V_F32S;
 }
+// LDS of an executor instantiation: the largest item of its variants.
+constexpr int vset_lds(int vm) {
+    int b = 0;
+    if (vm & V_F64S) b = cmax(b, cmax(cmax(LDS_GEMM_F64, LDS_POTRF), LDS_TRSM));
+    if (vm & V_F32S) b = cmax(b, cmax(LDS_POTRF, LDS_TRSM_F32));
+    if (vm & V_G32) b = cmax(b, LDS_GEMM_F32);
+    if (vm & V_GBF16) b = cmax(b, LDS_GEMM_BF16_1);
+    if (vm & V_GBF16X3) b = cmax(b, LDS_GEMM_BF16);
+    return b;
+}
+// Workgroups per CU an instantiation is compiled for: two when two of them fit the CU's
+// 160 KB of LDS (registers are then capped at 256 per lane, VGPRs and AGPRs together).
+constexpr int vset_occupancy(int vm) { return 2 * (vset_lds(vm) + 1024) <= 160 * 1024 ? 2 : 1; }
 inline int vset_index(int needed) {
     for (int q = 0; q < N_VSETS; q++)
         if ((needed & ~VSETS[q]) == 0) return q;
@@ -72,15 +95,23 @@ inline int vset_index(int needed) {
 // POTRF and TRSM items are called out of line from the executors: a kernel holding them
 // inline is register-allocated for their largest live set and spills in the GEMM loop,
 // which runs far more often. The call saves the callee's registers once per item.
-template <typename TS>
+// One instance per executor instantiation (VM): a callee shared by kernels compiled for
+// different occupancies would be register-allocated for the lowest of them.
+// POTRF and TRSM carry the critical path: their waves run at raised issue priority, so
+// that a GEMM item sharing the CU (two workgroups per CU) does not halve their MFMA share.
+template <typename TS, int VM>
 __device__ __attribute__((noinline)) void potrf_call(TS* A, long lda, int n, double* dinv, int* info, int col_off,
                                                      char* lds) {
+    __builtin_amdgcn_s_setprio(3);
     item_potrf<TS>(A, lda, n, dinv, info, col_off, lds);
+    __builtin_amdgcn_s_setprio(0);
 }
-template <typename TS, typename TC>
+template <typename TS, typename TC, int VM>
 __device__ __attribute__((noinline)) void trsm_call(const TS* L, long ldl, const double* dinv, TS* X, long ldx, int m,
                                                     int n, int r0, char* lds) {
+    __builtin_amdgcn_s_setprio(3);
     item_trsm<TS, TC>(L, ldl, dinv, X, ldx, m, n, r0, lds);
+    __builtin_amdgcn_s_setprio(0);
 }
 
 // Execute work item `it` of task t. All 256 threads participate. Returns false
@@ -96,12 +127,12 @@ __device__ __forceinline__ bool exec_item(const DevTask& t, int it, const DevMat
             const int j = t.j, n = M.bsz[j];
             if constexpr ((VM & V_F64S) != 0)
                 if (f64s) {
-                    potrf_call<double>((double*)tile(j, j), n, n, M.dinv[j], M.info, M.boff[j], lds);
+                    potrf_call<double, VM>((double*)tile(j, j), n, n, M.dinv[j], M.info, M.boff[j], lds);
                     return true;
                 }
             if constexpr ((VM & V_F32S) != 0)
                 if (!f64s) {
-                    potrf_call<float>((float*)tile(j, j), n, n, M.dinv[j], M.info, M.boff[j], lds);
+                    potrf_call<float, VM>((float*)tile(j, j), n, n, M.dinv[j], M.info, M.boff[j], lds);
                     return true;
                 }
             return false;
@@ -110,13 +141,13 @@ __device__ __forceinline__ bool exec_item(const DevTask& t, int it, const DevMat
             const int i = t.i, j = t.j, m = M.bsz[i], n = M.bsz[j];
             if constexpr ((VM & V_F64S) != 0)
                 if (f64s) {
-                    trsm_call<double, double>((const double*)tile(j, j), n, M.dinv[j], (double*)tile(i, j), m, m, n,
+                    trsm_call<double, double, VM>((const double*)tile(j, j), n, M.dinv[j], (double*)tile(i, j), m, m, n,
                                               it * TRSM_BM, lds);
                     return true;
                 }
             if constexpr ((VM & V_F32S) != 0)
                 if (!f64s) {
-                    trsm_call<float, float>((const float*)tile(j, j), n, M.dinv[j], (float*)tile(i, j), m, m, n,
+                    trsm_call<float, float, VM>((const float*)tile(j, j), n, M.dinv[j], (float*)tile(i, j), m, m, n,
                                             it * TRSM_BM, lds);
                     return true;
                 }
@@ -132,25 +163,25 @@ __device__ __forceinline__ bool exec_item(const DevTask& t, int it, const DevMat
             const int r0 = rb * GEMM_BM, c0 = cb * GEMM_BM;
             if constexpr ((VM & V_F64S) != 0)
                 if (t.prec == P_F64) {
-                    item_gemm<double, P_F64>((const double*)tile(i, k), m, (const double*)tile(j, k), n,
+                    item_gemm<double, P_F64, true>((const double*)tile(i, k), m, (const double*)tile(j, k), n,
                                              (double*)tile(i, j), m, m, n, K, r0, c0, syrk, -1.0, 1.0, lds);
                     return true;
                 }
             if constexpr ((VM & V_G32) != 0)
                 if (t.prec == P_F32) {
-                    item_gemm<float, P_F32>((const float*)tile(i, k), m, (const float*)tile(j, k), n,
+                    item_gemm<float, P_F32, true>((const float*)tile(i, k), m, (const float*)tile(j, k), n,
                                             (float*)tile(i, j), m, m, n, K, r0, c0, syrk, -1.0, 1.0, lds);
                     return true;
                 }
             if constexpr ((VM & V_GBF16) != 0)
                 if (t.prec == P_BF16) {
-                    item_gemm<float, P_BF16>((const float*)tile(i, k), m, (const float*)tile(j, k), n,
+                    item_gemm<float, P_BF16, true>((const float*)tile(i, k), m, (const float*)tile(j, k), n,
                                              (float*)tile(i, j), m, m, n, K, r0, c0, syrk, -1.0, 1.0, lds);
                     return true;
                 }
             if constexpr ((VM & V_GBF16X3) != 0)
                 if (t.prec == P_BF16X3) {
-                    item_gemm<float, P_BF16X3>((const float*)tile(i, k), m, (const float*)tile(j, k), n,
+                    item_gemm<float, P_BF16X3, true>((const float*)tile(i, k), m, (const float*)tile(j, k), n,
                                                (float*)tile(i, j), m, m, n, K, r0, c0, syrk, -1.0, 1.0, lds);
                     return true;
                 }

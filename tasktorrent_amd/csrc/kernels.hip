@@ -11,7 +11,7 @@ namespace ttad {
 // prefix[t] = first global item of task t (exclusive scan of n_items).
 // ---------------------------------------------------------------------------
 template <int VM>
-__global__ __launch_bounds__(NTHREADS) void k_items(const DevMat* mats, const DevTask* tasks, const int32_t* prefix,
+__global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_items(const DevMat* mats, const DevTask* tasks, const int32_t* prefix,
                                                     int n_tasks, int32_t* err) {
     extern __shared__ __align__(16) char lds[];
     const int g = blockIdx.x;
@@ -26,15 +26,19 @@ __global__ __launch_bounds__(NTHREADS) void k_items(const DevMat* mats, const De
 }
 
 // ---------------------------------------------------------------------------
-// Persistent DAG executor (replaces the Threadpool worker loop). Every workgroup loops:
-//   wave 0 reads the heads/tails of the priority buckets 64 at a time (one per lane,
-//   <= 256 buckets) and claims the head entry of the highest-priority non-empty bucket:
-//   with a fetch-add when the bucket holds more entries than there are workers
-//   (no claim can then pass the tail), else with a CAS (losers back off a random
+// Persistent DAG executor (replaces the Threadpool worker loop). Work queues are ordered
+// by priority; a queue is either shared or homed on one XCD (the GEMM/SYRK items of a
+// priority bucket are split by output tile row, so that the items of one tile update,
+// and the updates that share its A operand, meet in one XCD's L2). Every workgroup loops:
+//   wave 0 reads the heads/tails of the queues 64 at a time (one per lane) and claims
+//   the head entry of the first non-empty queue that is shared or homed on its own XCD;
+//   when there is none it takes the first non-empty queue of another XCD (stealing).
+//   The claim is a fetch-add when the queue holds more entries than twice the number
+//   of workers (no claim can then pass the tail), else a CAS (losers back off a random
 //   interval); the workgroup runs that work item, then thread 0 decrements the
 //   task's remaining-items counter; the last item of a task decrements its
 //   successors' dependency counters and pushes every item of a successor that
-//   reaches zero into its bucket. A worker exits when every bucket is fully claimed.
+//   reaches zero into its queue. A worker exits when every queue is fully claimed.
 // Memory ordering: item outputs are released (agent scope) before the counter
 // RMWs; queue entries are published with release stores and read with acquire
 // loads, so a consumer on another XCD sees the producer's tiles.
@@ -56,11 +60,12 @@ __device__ __forceinline__ void push_task(const DevGraph& g, int s) {
 }
 
 template <int VM>
-__global__ __launch_bounds__(NTHREADS) void k_persistent(DevGraph g) {
+__global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(DevGraph g) {
     extern __shared__ __align__(16) char lds[];
     __shared__ int64_t s_item;
     const int tid = threadIdx.x;
     const int margin = 2 * (int)gridDim.x;  // more queued items than this: a fetch-add claim cannot overshoot
+    const int my_xcd = xcc_id();
     uint32_t rng = blockIdx.x * 2654435761u + 0x9e3779b9u;
     for (;;) {
         if (tid < 64) {
@@ -68,20 +73,29 @@ __global__ __launch_bounds__(NTHREADS) void k_persistent(DevGraph g) {
             int64_t got = -1;
             const int64_t t_start = (int64_t)wall_clock64();
             for (;;) {
-                int fb = -1, fh = 0, ft = 0;
+                int fb = -1, fh = 0, ft = 0;  // queue to claim from
+                int sb = -1, sh = 0, st = 0;  // first non-empty queue of any XCD (to steal from)
                 bool any_open = false;
-                for (int base = 0; base < g.n_buckets; base += 64) {  // groups of 64 buckets, best first
+                for (int base = 0; base < g.n_buckets; base += 64) {  // groups of 64 queues, best first
                     const int b = base + tid;
                     int h = 0, t = 0;
-                    bool ne = false, open = false;
+                    bool ne = false, open = false, mine = false;
                     if (b < g.n_buckets) {
                         h = ld_relaxed(g.q_head + b);
                         t = ld_relaxed(g.q_tail + b);
                         ne = h < t;
                         open = h < g.q_base[b + 1] - g.q_base[b];
+                        const int home = g.q_home[b];
+                        mine = ne && (home < 0 || home == my_xcd);
                     }
-                    const unsigned long long mask = __ballot(ne);
+                    const unsigned long long mask = __ballot(mine), any = __ballot(ne);
                     any_open |= __ballot(open) != 0;
+                    if (sb < 0 && any) {
+                        const int l = __ffsll((long long)any) - 1;
+                        sb = base + l;
+                        sh = __shfl(h, l);
+                        st = __shfl(t, l);
+                    }
                     if (mask) {
                         const int l = __ffsll((long long)mask) - 1;
                         fb = base + l;
@@ -89,6 +103,11 @@ __global__ __launch_bounds__(NTHREADS) void k_persistent(DevGraph g) {
                         ft = __shfl(t, l);
                         break;
                     }
+                }
+                if (fb < 0 && sb >= 0) {  // nothing for this XCD: steal
+                    fb = sb;
+                    fh = sh;
+                    ft = st;
                 }
                 const bool all_claimed = fb < 0 && !any_open;
                 if (fb >= 0) {
@@ -209,7 +228,7 @@ __global__ void k_graph_reset_b(DevGraph g) {
 // Raw tile kernels behind the BLAS-shaped C-ABI entry points.
 // ---------------------------------------------------------------------------
 template <typename TS, int PREC>
-__global__ __launch_bounds__(NTHREADS) void k_raw_gemm(const TS* A, long lda, const TS* B, long ldb, TS* C, long ldc,
+__global__ __launch_bounds__(NTHREADS, 2 * (gemm_item_lds<TS, PREC>() + 1024) <= 160 * 1024 ? 2 : 1) void k_raw_gemm(const TS* A, long lda, const TS* B, long ldb, TS* C, long ldc,
                                                        int m, int n, int K, int syrk, double alpha, double beta) {
     extern __shared__ __align__(16) char lds[];
     int rb, cb;
@@ -305,27 +324,21 @@ static hipError_t set_lds(const void* f, int bytes) {
 
 hipError_t kernels_init() {
     hipError_t e = hipSuccess;
-#define TTAD_SETLDS(fn)                                          \
-    do {                                                         \
-        e = set_lds((const void*)(fn), LDS_ITEM_MAX);            \
-        if (e != hipSuccess) return e;                           \
-    } while (0)
-    TTAD_SETLDS(k_items<VSETS[0]>);
-    TTAD_SETLDS(k_items<VSETS[1]>);
-    TTAD_SETLDS(k_items<VSETS[2]>);
-    TTAD_SETLDS(k_items<VSETS[3]>);
-    TTAD_SETLDS(k_items<VSETS[4]>);
-    TTAD_SETLDS(k_persistent<VSETS[0]>);
-    TTAD_SETLDS(k_persistent<VSETS[1]>);
-    TTAD_SETLDS(k_persistent<VSETS[2]>);
-    TTAD_SETLDS(k_persistent<VSETS[3]>);
-    TTAD_SETLDS(k_persistent<VSETS[4]>);
-#undef TTAD_SETLDS
 #define TTAD_SETLDS2(fn, bytes)                      \
     do {                                             \
         e = set_lds((const void*)(fn), (bytes));     \
         if (e != hipSuccess) return e;               \
     } while (0)
+    TTAD_SETLDS2(k_items<VSETS[0]>, vset_lds(VSETS[0]));
+    TTAD_SETLDS2(k_items<VSETS[1]>, vset_lds(VSETS[1]));
+    TTAD_SETLDS2(k_items<VSETS[2]>, vset_lds(VSETS[2]));
+    TTAD_SETLDS2(k_items<VSETS[3]>, vset_lds(VSETS[3]));
+    TTAD_SETLDS2(k_items<VSETS[4]>, vset_lds(VSETS[4]));
+    TTAD_SETLDS2(k_persistent<VSETS[0]>, vset_lds(VSETS[0]));
+    TTAD_SETLDS2(k_persistent<VSETS[1]>, vset_lds(VSETS[1]));
+    TTAD_SETLDS2(k_persistent<VSETS[2]>, vset_lds(VSETS[2]));
+    TTAD_SETLDS2(k_persistent<VSETS[3]>, vset_lds(VSETS[3]));
+    TTAD_SETLDS2(k_persistent<VSETS[4]>, vset_lds(VSETS[4]));
     TTAD_SETLDS2((k_raw_gemm<double, P_F64>), (gemm_item_lds<double, P_F64>()));
     TTAD_SETLDS2((k_raw_gemm<float, P_F32>), (gemm_item_lds<float, P_F32>()));
     TTAD_SETLDS2((k_raw_gemm<float, P_BF16>), (gemm_item_lds<float, P_BF16>()));
@@ -348,28 +361,35 @@ static const void* persistent_fn(int vset) {
     }
 }
 
-int persistent_blocks_per_cu() {
-    int best = 1 << 30;  // the smallest over the variants: one worker count fits every graph
-    for (int q = 0; q < N_VSETS; q++) {
-        int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, persistent_fn(q), NTHREADS, LDS_ITEM_MAX) !=
-            hipSuccess)
-            return 1;
-        best = std::min(best, nb);
+static int vset_lds_of(int vset) {
+    switch (vset) {
+        case 0: return vset_lds(VSETS[0]);
+        case 1: return vset_lds(VSETS[1]);
+        case 2: return vset_lds(VSETS[2]);
+        case 3: return vset_lds(VSETS[3]);
+        default: return vset_lds(VSETS[4]);
     }
-    return best > 0 ? best : 1;
+}
+
+int persistent_blocks_per_cu(int vset) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, persistent_fn(vset), NTHREADS, vset_lds_of(vset)) !=
+        hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
 }
 
 void launch_items(int vset, const DevMat* mats, const DevTask* tasks, const int32_t* prefix, int n_tasks,
                   int n_items, int32_t* err, hipStream_t s) {
     if (n_items <= 0) return;
     const dim3 grid(n_items), block(NTHREADS);
+    const int lds = vset_lds_of(vset);
     switch (vset) {
-        case 0: hipLaunchKernelGGL(k_items<VSETS[0]>, grid, block, LDS_ITEM_MAX, s, mats, tasks, prefix, n_tasks, err); break;
-        case 1: hipLaunchKernelGGL(k_items<VSETS[1]>, grid, block, LDS_ITEM_MAX, s, mats, tasks, prefix, n_tasks, err); break;
-        case 2: hipLaunchKernelGGL(k_items<VSETS[2]>, grid, block, LDS_ITEM_MAX, s, mats, tasks, prefix, n_tasks, err); break;
-        case 3: hipLaunchKernelGGL(k_items<VSETS[3]>, grid, block, LDS_ITEM_MAX, s, mats, tasks, prefix, n_tasks, err); break;
-        default: hipLaunchKernelGGL(k_items<VSETS[4]>, grid, block, LDS_ITEM_MAX, s, mats, tasks, prefix, n_tasks, err); break;
+        case 0: hipLaunchKernelGGL(k_items<VSETS[0]>, grid, block, lds, s, mats, tasks, prefix, n_tasks, err); break;
+        case 1: hipLaunchKernelGGL(k_items<VSETS[1]>, grid, block, lds, s, mats, tasks, prefix, n_tasks, err); break;
+        case 2: hipLaunchKernelGGL(k_items<VSETS[2]>, grid, block, lds, s, mats, tasks, prefix, n_tasks, err); break;
+        case 3: hipLaunchKernelGGL(k_items<VSETS[3]>, grid, block, lds, s, mats, tasks, prefix, n_tasks, err); break;
+        default: hipLaunchKernelGGL(k_items<VSETS[4]>, grid, block, lds, s, mats, tasks, prefix, n_tasks, err); break;
     }
 }
 
@@ -380,12 +400,13 @@ void launch_graph_reset(const DevGraph& g, int n_slots, hipStream_t s) {
 
 void launch_persistent(int vset, const DevGraph& g, int n_workers, hipStream_t s) {
     const dim3 grid(n_workers), block(NTHREADS);
+    const int lds = vset_lds_of(vset);
     switch (vset) {
-        case 0: hipLaunchKernelGGL(k_persistent<VSETS[0]>, grid, block, LDS_ITEM_MAX, s, g); break;
-        case 1: hipLaunchKernelGGL(k_persistent<VSETS[1]>, grid, block, LDS_ITEM_MAX, s, g); break;
-        case 2: hipLaunchKernelGGL(k_persistent<VSETS[2]>, grid, block, LDS_ITEM_MAX, s, g); break;
-        case 3: hipLaunchKernelGGL(k_persistent<VSETS[3]>, grid, block, LDS_ITEM_MAX, s, g); break;
-        default: hipLaunchKernelGGL(k_persistent<VSETS[4]>, grid, block, LDS_ITEM_MAX, s, g); break;
+        case 0: hipLaunchKernelGGL(k_persistent<VSETS[0]>, grid, block, lds, s, g); break;
+        case 1: hipLaunchKernelGGL(k_persistent<VSETS[1]>, grid, block, lds, s, g); break;
+        case 2: hipLaunchKernelGGL(k_persistent<VSETS[2]>, grid, block, lds, s, g); break;
+        case 3: hipLaunchKernelGGL(k_persistent<VSETS[3]>, grid, block, lds, s, g); break;
+        default: hipLaunchKernelGGL(k_persistent<VSETS[4]>, grid, block, lds, s, g); break;
     }
 }
 

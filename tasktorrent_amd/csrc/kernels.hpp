@@ -8,7 +8,8 @@ namespace ttad {
 struct DevGraph;
 
 hipError_t kernels_init();
-int persistent_blocks_per_cu();
+// Resident persistent workers per CU of an executor instantiation (index into the variant sets).
+int persistent_blocks_per_cu(int vset);
 // vset: index into the compiled item-variant sets (vset_index of the variants a graph needs).
 void launch_items(int vset, const DevMat* mats, const DevTask* tasks, const int32_t* prefix, int n_tasks,
                   int n_items, int32_t* err, hipStream_t s);

@@ -58,7 +58,8 @@ struct ttad_context {
     int n_cu = 0;
     long long hbm_bytes = 0;
     std::string name;
-    int workers_per_cu = 1;
+    int workers_per_cu = 1;            // of the FP64 executor
+    int vset_workers_per_cu[8] = {1, 1, 1, 1, 1, 1, 1, 1};  // per executor instantiation
     ttad::DevBuf scratch_dinv;  // inverse diagonal blocks for the raw tile entry points
     ttad::DevBuf scratch_info;  // one int
     // host-sequenced schedules, keyed by (matrix id, prec)
@@ -94,13 +95,14 @@ struct ttad_matrix {
 
 struct ttad_graph {
     ttad_context* ctx = nullptr;
-    int n_tasks = 0, n_buckets = 0, total_items = 0;
+    int n_tasks = 0, n_buckets = 0, total_items = 0;  // n_buckets counts work queues
     std::vector<ttad::DevTask> tasks;
     std::vector<int32_t> indegree, seeds, out_ptr, out_idx, q_base, bucket_items;
+    std::vector<int32_t> q_home;  // per queue: -1 shared, else the XCD whose workers it feeds first
     std::vector<ttad_matrix*> mats;
     bool seeds_dirty = true, mats_dirty = true, running = false;
     ttad::DevBuf d_tasks, d_out_ptr, d_out_idx, d_deps0, d_tail0, d_init, d_deps, d_items_left, d_q_head, d_q_tail,
-        d_q_base, d_q_items, d_done, d_abort, d_mats, d_trace, d_trace_cursor;
+        d_q_base, d_q_home, d_q_items, d_done, d_abort, d_mats, d_trace, d_trace_cursor;
     int n_init = 0;
     int workers = 0;
     int vset = 4;  // executor instantiation (index into the item-variant sets) covering the graph

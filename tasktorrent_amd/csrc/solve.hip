@@ -10,10 +10,22 @@
 #include "solve_ops.hpp"
 
 namespace ttad {
-// Priority kind of the factorizations inside ttad_posv_mixed: 0, the miniapp's default
-// (2d_cholesky.cpp `--kind`), only orders POTRF > TRSM > GEMM; with three buckets the GEMM
-// bucket always holds more items than there are workers, so claims are contention-free.
-constexpr int MIXED_PRIO_KIND = 0;
+// Integer tuning override from the environment, clamped to [lo, hi].
+static int env_int(const char* name, int dflt, int lo, int hi) {
+    const char* v = getenv(name);
+    if (!v || !*v) return dflt;
+    return std::min(hi, std::max(lo, atoi(v)));
+}
+
+// Priority kind of the factorizations inside ttad_posv_mixed (2d_cholesky.cpp `--kind`):
+// 2, critical path first (every task of panel column j at 9(nb-j)-c). Measured at n = 32768
+// with two workers per CU: bf16x3 time-to-solution 293 ms (kind 2) vs 335 ms (kind 0,
+// POTRF > TRSM > GEMM only), 308 ms (kind 3), 350 ms (kind 1).
+constexpr int MIXED_PRIO_KIND_DEFAULT = 2;
+static int mixed_prio_kind() {  // TTAD_MIXED_PRIO overrides it (tuning)
+    static const int k = env_int("TTAD_MIXED_PRIO", MIXED_PRIO_KIND_DEFAULT, 0, 3);
+    return k;
+}
 
 
 // Per-matrix 64-row block table (built on first use).
@@ -37,12 +49,6 @@ static int ensure_sweep_tab(ttad_matrix* M) {
 }
 
 // Sweep waiting policy; TTAD_SWEEP_NEAR / TTAD_SWEEP_BATCH override the defaults (tuning).
-static int env_int(const char* name, int dflt, int lo, int hi) {
-    const char* v = getenv(name);
-    if (!v || !*v) return dflt;
-    return std::min(hi, std::max(lo, atoi(v)));
-}
-
 static SweepTab tab_of(const ttad_matrix* M) {
     static const int near = env_int("TTAD_SWEEP_NEAR", SWEEP_NEAR, 1, 1 << 20);
     static const int batch = env_int("TTAD_SWEEP_BATCH", SWEEP_BATCH, 1, 1 << 20);
@@ -294,7 +300,7 @@ int ttad_posv_mixed(ttad_context* ctx, ttad_matrix* A, int prec, int nrhs, const
     if (!W) return rc;
     if ((rc = ttad_matrix_copy(W, A, s))) return rc;
     int info = 0;
-    rc = ttad_potrf_tiled(ctx, W, TTAD_SCHED_PERSISTENT, MIXED_PRIO_KIND, prec, s, &info);
+    rc = ttad_potrf_tiled(ctx, W, TTAD_SCHED_PERSISTENT, mixed_prio_kind(), prec, s, &info);
     loc.factor_s = secs();
     loc.info = info;
     int iters = 0;
@@ -311,7 +317,7 @@ int ttad_posv_mixed(ttad_context* ctx, ttad_matrix* A, int prec, int nrhs, const
         ttad_matrix* W64 = working(ctx, A, P_F64, rc);
         if (!W64) return rc;
         if ((rc = ttad_matrix_copy(W64, A, s))) return rc;
-        rc = ttad_potrf_tiled(ctx, W64, TTAD_SCHED_PERSISTENT, MIXED_PRIO_KIND, P_F64, s, &info);
+        rc = ttad_potrf_tiled(ctx, W64, TTAD_SCHED_PERSISTENT, mixed_prio_kind(), P_F64, s, &info);
         loc.info = info;
         if (rc == TTAD_OK) {
             hipLaunchKernelGGL(k_zero_cols, dim3(256), dim3(256), 0, s, X, ldx, A->n, nrhs);

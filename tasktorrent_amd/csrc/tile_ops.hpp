@@ -9,6 +9,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
+#include <type_traits>
 #include "ttad_types.hpp"
 
 namespace ttad {
@@ -44,6 +45,19 @@ template <> struct Mfma<float> {
 
 template <typename T> __device__ __forceinline__ T zero_val() { return T(0); }
 
+// 16-byte register vectors of a storage type (aligned: LDS side; u: global side, element-aligned).
+template <typename T> struct V16;
+template <> struct V16<double> {
+    static constexpr int N = 2;
+    typedef double t __attribute__((ext_vector_type(2)));
+    typedef double u __attribute__((ext_vector_type(2), aligned(8)));
+};
+template <> struct V16<float> {
+    static constexpr int N = 4;
+    typedef float t __attribute__((ext_vector_type(4)));
+    typedef float u __attribute__((ext_vector_type(4), aligned(4)));
+};
+
 // ---------------------------------------------------------------------------
 // acc (BM x BN) += A(BM x K) * B(BN x K)^T with A, B column-major (lda, ldb),
 // rows >= mv of A and rows >= nv of B read as zero. TA/TB are storage types,
@@ -51,21 +65,34 @@ template <typename T> __device__ __forceinline__ T zero_val() { return T(0); }
 // 2x2 waves, each wave owns a (BM/2) x (BN/2) block of 16x16 MFMA tiles.
 // K is staged through two LDS buffers in chunks of BK: while the MFMAs consume
 // chunk c, chunk c+1 goes from registers to the other buffer and the global
-// loads of chunk c+2 are in flight (one barrier per chunk). A chunk's MFMA time
-// (BK=32 FP64: 8 k-steps x 16 MFMAs per wave) is what hides the load latency.
+// loads of chunk c+2 are in flight (one barrier per chunk).
+// VEC: each lane loads 16 bytes (2 FP64 / 4 FP32 consecutive rows of a column) and
+//      stores them to LDS in one instruction (TA = TB = TC).
+// SWAP: the MFMA takes B's block as its first operand, so that a lane's results lie
+//      along a column of the output (row = lane % 16): C is then read and written in
+//      16-row segments instead of 16 scattered columns.
 // ---------------------------------------------------------------------------
-template <typename TA, typename TB, typename TC, int BM, int BN, int BK, int DEPTH = 1>
+template <typename TA, typename TB, typename TC, int BM, int BN, int BK, bool VEC = false, bool SWAP = false>
 struct GemmNT {
-    static constexpr int PA = BM + 2;
-    static constexpr int PB = BN + 2;
+    static constexpr int NVA = VEC ? V16<TA>::N : 1;  // rows per lane load
+    static constexpr int NVB = VEC ? V16<TB>::N : 1;
+    static constexpr int PA = BM + (NVA > 2 ? NVA : 2);  // LDS row pitch (16-byte multiple when VEC)
+    static constexpr int PB = BN + (NVB > 2 ? NVB : 2);
     static constexpr int WM = BM / 32;
     static constexpr int WN = BN / 32;
-    static constexpr int EA = BM * BK / NTHREADS;
-    static constexpr int EB = BN * BK / NTHREADS;
+    static constexpr int LA = BM / NVA;          // lanes per column of the A chunk
+    static constexpr int LB = BN / NVB;
+    static constexpr int KS_A = NTHREADS / LA;   // columns per load round
+    static constexpr int KS_B = NTHREADS / LB;
+    static constexpr int EA = BK / KS_A;         // loads per thread per chunk
+    static constexpr int EB = BK / KS_B;
     static constexpr int STAGE = BK * (PA + PB);  // TC elements of one LDS buffer
     static constexpr int LDS_BYTES = 2 * STAGE * (int)sizeof(TC);
     typedef typename Mfma<TC>::acc_t acc_t;
-    static_assert(EA * NTHREADS == BM * BK && EB * NTHREADS == BN * BK, "tile/thread mismatch");
+    typedef std::conditional_t<VEC, typename V16<TA>::t, TA> RA;
+    typedef std::conditional_t<VEC, typename V16<TB>::t, TB> RB;
+    static_assert(NTHREADS % LA == 0 && NTHREADS % LB == 0 && BK % KS_A == 0 && BK % KS_B == 0, "tile/thread mismatch");
+    static_assert(!VEC || (std::is_same<TA, TC>::value && std::is_same<TB, TC>::value), "VEC stages without conversion");
     static_assert(BK % 4 == 0, "BK is a multiple of the MFMA k-step");
 
     static __device__ __forceinline__ void zero(acc_t (&acc)[WM][WN]) {
@@ -77,37 +104,58 @@ struct GemmNT {
     // Element (row, col) of the output block held by this lane in acc[a][b][r].
     static __device__ __forceinline__ int row_of(int a, int r) {
         const int lane = threadIdx.x & 63, wr = (threadIdx.x >> 6) >> 1;
-        return wr * (WM * 16) + a * 16 + Mfma<TC>::row(lane, r);
+        return wr * (WM * 16) + a * 16 + (SWAP ? (lane & 15) : Mfma<TC>::row(lane, r));
     }
-    static __device__ __forceinline__ int col_of(int b) {
+    static __device__ __forceinline__ int col_of(int b, int r) {
         const int lane = threadIdx.x & 63, wc = (threadIdx.x >> 6) & 1;
-        return wc * (WN * 16) + b * 16 + (lane & 15);
+        return wc * (WN * 16) + b * 16 + (SWAP ? Mfma<TC>::row(lane, r) : (lane & 15));
     }
 
-    // Each thread stages a fixed row (tid % BM) and every (NTHREADS/BM)-th k of a chunk:
-    // one wave instruction reads 64 consecutive rows of one column (coalesced).
-    static constexpr int KS_A = NTHREADS / BM;
-    static constexpr int KS_B = NTHREADS / BN;
-    static_assert(NTHREADS % BM == 0 && NTHREADS % BN == 0, "BM, BN must divide the workgroup");
-
-    template <typename T, int E, int KS>
-    static __device__ __forceinline__ void load_chunk(T (&reg)[E], const T* __restrict__ base, long ld, bool row_ok,
-                                                      int kfirst, int K) {
+    // Thread tid stages rows NV*(tid % L) .. +NV-1 of every (NTHREADS/L)-th column of a chunk:
+    // one wave instruction reads whole columns (coalesced).
+    template <typename T, typename R, int E, int NV, int L, int KS>
+    static __device__ __forceinline__ void load_chunk(R (&reg)[E], const T* __restrict__ base, long ld, int nvalid,
+                                                      int k0, int K) {
+        const int row = NV * ((int)threadIdx.x % L), kc = (int)threadIdx.x / L;
+        if (nvalid >= L * NV && k0 + KS * E <= K) {  // whole chunk inside (uniform): no guards
+#pragma unroll
+            for (int s = 0; s < E; s++) {
+                const T* p = base + row + (long)(k0 + kc + s * KS) * ld;
+                if constexpr (NV == 1) reg[s] = *p;
+                else reg[s] = *reinterpret_cast<const typename V16<T>::u*>(p);
+            }
+            return;
+        }
 #pragma unroll
         for (int s = 0; s < E; s++) {
-            const int kk = kfirst + s * KS;
-            reg[s] = (row_ok && kk < K) ? base[(long)kk * ld] : T(0);
+            const int kk = k0 + kc + s * KS;
+            const T* p = base + row + (long)kk * ld;
+            if constexpr (NV == 1) {
+                reg[s] = (row < nvalid && kk < K) ? *p : T(0);
+            } else {
+#pragma unroll
+                for (int q = 0; q < NV; q++) reg[s][q] = (kk < K && row + q < nvalid) ? p[q] : T(0);
+            }
         }
     }
-
-    static __device__ __forceinline__ void store_chunk(TC* __restrict__ S, const TA (&ra)[EA], const TB (&rb)[EB],
-                                                       int ra_row, int ra_k, int rb_row, int rb_k) {
-        TC* As = S;
-        TC* Bs = S + BK * PA;
+    template <typename R, int E, int NV, int L, int KS, int P>
+    static __device__ __forceinline__ void store_chunk(TC* __restrict__ S, const R (&reg)[E]) {
+        const int row = NV * ((int)threadIdx.x % L), kc = (int)threadIdx.x / L;
 #pragma unroll
-        for (int s = 0; s < EA; s++) As[(ra_k + s * KS_A) * PA + ra_row] = (TC)ra[s];
-#pragma unroll
-        for (int s = 0; s < EB; s++) Bs[(rb_k + s * KS_B) * PB + rb_row] = (TC)rb[s];
+        for (int s = 0; s < E; s++) {
+            TC* q = S + (kc + s * KS) * P + row;
+            if constexpr (NV == 1) *q = (TC)reg[s];
+            else *reinterpret_cast<R*>(q) = reg[s];
+        }
+    }
+    static __device__ __forceinline__ void load(RA (&ra)[EA], RB (&rb)[EB], const TA* A, long lda, int mv, const TB* B,
+                                                long ldb, int nv, int k0, int K) {
+        load_chunk<TA, RA, EA, NVA, LA, KS_A>(ra, A, lda, mv, k0, K);
+        load_chunk<TB, RB, EB, NVB, LB, KS_B>(rb, B, ldb, nv, k0, K);
+    }
+    static __device__ __forceinline__ void store(TC* __restrict__ S, const RA (&ra)[EA], const RB (&rb)[EB]) {
+        store_chunk<RA, EA, NVA, LA, KS_A, PA>(S, ra);
+        store_chunk<RB, EB, NVB, LB, KS_B, PB>(S + BK * PA, rb);
     }
 
     static __device__ __forceinline__ void compute(acc_t (&acc)[WM][WN], const TC* __restrict__ S, int lane, int wr,
@@ -125,93 +173,32 @@ struct GemmNT {
 #pragma unroll
             for (int a = 0; a < WM; a++)
 #pragma unroll
-                for (int b = 0; b < WN; b++) acc[a][b] = Mfma<TC>::mma(av[a], bv[b], acc[a][b]);
-        }
-    }
-
-    // DEPTH = 2: two register sets, so that a chunk's global loads are issued two chunks
-    // before its LDS store (twice the latency budget, twice the prefetch registers).
-    static __device__ __forceinline__ void run2(acc_t (&acc)[WM][WN], const TA* __restrict__ A, long lda, int mv,
-                                                const TB* __restrict__ B, long ldb, int nv, int K, char* lds) {
-        TC* S0 = reinterpret_cast<TC*>(lds);
-        TC* S1 = S0 + STAGE;
-        const int tid = threadIdx.x;
-        const int lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
-        const int ra_row = tid % BM, ra_k = tid / BM;
-        const int rb_row = tid % BN, rb_k = tid / BN;
-        const bool a_ok = ra_row < mv, b_ok = rb_row < nv;
-        const TA* Ab = A + ra_row;
-        const TB* Bb = B + rb_row;
-        const int nchunks = (K + BK - 1) / BK;
-        __syncthreads();
-        if (nchunks == 0) return;
-        TA ra0[EA], ra1[EA];
-        TB rb0[EB], rb1[EB];
-        auto load = [&](TA (&ra)[EA], TB (&rb)[EB], int c) {
-            load_chunk<TA, EA, KS_A>(ra, Ab, lda, a_ok, c * BK + ra_k, K);
-            load_chunk<TB, EB, KS_B>(rb, Bb, ldb, b_ok, c * BK + rb_k, K);
-        };
-        load(ra0, rb0, 0);
-        store_chunk(S0, ra0, rb0, ra_row, ra_k, rb_row, rb_k);
-        if (nchunks > 2) load(ra0, rb0, 2);
-        if (nchunks > 1) load(ra1, rb1, 1);
-        __syncthreads();
-        for (int c = 0; c < nchunks; c += 2) {
-            // even chunk c in S0; chunk c+1 waits in set 1, chunk c+2 in set 0
-            compute(acc, S0, lane, wr, wc);
-            if (c + 1 < nchunks) {
-                store_chunk(S1, ra1, rb1, ra_row, ra_k, rb_row, rb_k);
-                if (c + 3 < nchunks) load(ra1, rb1, c + 3);
-            }
-            __syncthreads();
-            if (c + 1 >= nchunks) break;
-            compute(acc, S1, lane, wr, wc);
-            if (c + 2 < nchunks) {
-                store_chunk(S0, ra0, rb0, ra_row, ra_k, rb_row, rb_k);
-                if (c + 4 < nchunks) load(ra0, rb0, c + 4);
-            }
-            __syncthreads();
+                for (int b = 0; b < WN; b++)
+                    acc[a][b] = SWAP ? Mfma<TC>::mma(bv[b], av[a], acc[a][b]) : Mfma<TC>::mma(av[a], bv[b], acc[a][b]);
         }
     }
 
     // Starts and ends with a workgroup barrier: the caller may use `lds` before and after.
     static __device__ __forceinline__ void run(acc_t (&acc)[WM][WN], const TA* __restrict__ A, long lda, int mv,
-                               const TB* __restrict__ B, long ldb, int nv, int K, char* lds) {
-        if constexpr (DEPTH == 2) {
-            run2(acc, A, lda, mv, B, ldb, nv, K, lds);
-            return;
-        }
+                                               const TB* __restrict__ B, long ldb, int nv, int K, char* lds) {
         TC* S0 = reinterpret_cast<TC*>(lds);
         const int tid = threadIdx.x;
         const int lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
-        const int ra_row = tid % BM, ra_k = tid / BM;
-        const int rb_row = tid % BN, rb_k = tid / BN;
-        const bool a_ok = ra_row < mv, b_ok = rb_row < nv;
-        const TA* Ab = A + ra_row;
-        const TB* Bb = B + rb_row;
         const int nchunks = (K + BK - 1) / BK;
         __syncthreads();
         if (nchunks == 0) return;
-        TA ra[EA];
-        TB rb[EB];
-        load_chunk<TA, EA, KS_A>(ra, Ab, lda, a_ok, ra_k, K);
-        load_chunk<TB, EB, KS_B>(rb, Bb, ldb, b_ok, rb_k, K);
-        store_chunk(S0, ra, rb, ra_row, ra_k, rb_row, rb_k);
-        if (nchunks > 1) {
-            load_chunk<TA, EA, KS_A>(ra, Ab, lda, a_ok, BK + ra_k, K);
-            load_chunk<TB, EB, KS_B>(rb, Bb, ldb, b_ok, BK + rb_k, K);
-        }
+        RA ra[EA];
+        RB rb[EB];
+        load(ra, rb, A, lda, mv, B, ldb, nv, 0, K);
+        store(S0, ra, rb);
+        if (nchunks > 1) load(ra, rb, A, lda, mv, B, ldb, nv, BK, K);
         __syncthreads();
         for (int c = 0; c < nchunks; c++) {
             compute(acc, S0 + (c & 1) * STAGE, lane, wr, wc);
             if (c + 1 < nchunks) {
                 // the other buffer was last read in chunk c-1, before the previous barrier
-                store_chunk(S0 + ((c + 1) & 1) * STAGE, ra, rb, ra_row, ra_k, rb_row, rb_k);
-                if (c + 2 < nchunks) {
-                    const int k0 = (c + 2) * BK;
-                    load_chunk<TA, EA, KS_A>(ra, Ab, lda, a_ok, k0 + ra_k, K);
-                    load_chunk<TB, EB, KS_B>(rb, Bb, ldb, b_ok, k0 + rb_k, K);
-                }
+                store(S0 + ((c + 1) & 1) * STAGE, ra, rb);
+                if (c + 2 < nchunks) load(ra, rb, A, lda, mv, B, ldb, nv, (c + 2) * BK, K);
             }
             __syncthreads();
         }
@@ -237,10 +224,14 @@ __device__ __forceinline__ unsigned pack_bf16x2(float a, float b) {
     return __builtin_bit_cast(unsigned, __builtin_convertvector((f2_t){a, b}, bf2_t));
 }
 
-template <typename TA, int BM, int BN, bool SPLIT = false>
+template <typename TA, int BM, int BN, bool SPLIT = false, bool SWAP = false>
 struct GemmNTbf16 {
-    static constexpr int BK = 64;
-    static constexpr int PK = BK + 8;  // bf16 elements per LDS row (16-byte aligned, skewed)
+    // K-chunk: 64 (one plane), 32 with the hi/lo planes, so that both fit two workgroups per
+    // CU (72 KB of LDS). Rows of 72 elements (144 B) are read 16 bytes at a time; rows of 36
+    // (72 B, the split planes) 8 bytes at a time; both pitches spread 16 rows over distinct banks.
+    static constexpr int BK = SPLIT ? 32 : 64;
+    static constexpr int PK = SPLIT ? BK + 4 : BK + 8;  // bf16 elements per LDS row
+    static constexpr bool NARROW = (PK * 2) % 16 != 0;  // LDS accesses in 8-byte halves
     static constexpr int WM = BM / 32;
     static constexpr int WN = BN / 32;
     static constexpr int NP = SPLIT ? 2 : 1;  // hi (and lo) planes
@@ -260,11 +251,15 @@ struct GemmNTbf16 {
     }
     static __device__ __forceinline__ int row_of(int a, int r) {
         const int lane = threadIdx.x & 63, wr = (threadIdx.x >> 6) >> 1;
-        return wr * (WM * 16) + a * 16 + 4 * (lane >> 4) + r;
+        return wr * (WM * 16) + a * 16 + (SWAP ? (lane & 15) : 4 * (lane >> 4) + r);
     }
-    static __device__ __forceinline__ int col_of(int b) {
+    static __device__ __forceinline__ int col_of(int b, int r) {
         const int lane = threadIdx.x & 63, wc = (threadIdx.x >> 6) & 1;
-        return wc * (WN * 16) + b * 16 + (lane & 15);
+        return wc * (WN * 16) + b * 16 + (SWAP ? 4 * (lane >> 4) + r : (lane & 15));
+    }
+    static __device__ __forceinline__ f4_t mma(s8_t a, s8_t b, f4_t c) {
+        return SWAP ? __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a, c, 0, 0, 0)
+                    : __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
     }
 
     template <int G, int GS>
@@ -294,11 +289,31 @@ struct GemmNTbf16 {
                     l[q] = pack_bf16x2(x0 - __uint_as_float(hp << 16), x1 - __uint_as_float(hp & 0xffff0000u));
             }
             const int off = row * PK + (g0 + s * GS) * 8;
-            *reinterpret_cast<u4_t*>(hi_plane + off) = h;
-            if constexpr (SPLIT) *reinterpret_cast<u4_t*>(lo_plane + off) = l;
+            if constexpr (NARROW) {
+                typedef unsigned u2_t __attribute__((ext_vector_type(2)));
+                *reinterpret_cast<u2_t*>(hi_plane + off) = u2_t{h[0], h[1]};
+                *reinterpret_cast<u2_t*>(hi_plane + off + 4) = u2_t{h[2], h[3]};
+                if constexpr (SPLIT) {
+                    *reinterpret_cast<u2_t*>(lo_plane + off) = u2_t{l[0], l[1]};
+                    *reinterpret_cast<u2_t*>(lo_plane + off + 4) = u2_t{l[2], l[3]};
+                }
+            } else {
+                *reinterpret_cast<u4_t*>(hi_plane + off) = h;
+                if constexpr (SPLIT) *reinterpret_cast<u4_t*>(lo_plane + off) = l;
+            }
         }
     }
 
+    // 8 consecutive bf16 (one lane's k-group) from LDS
+    static __device__ __forceinline__ s8_t ld8(const unsigned short* p) {
+        if constexpr (NARROW) {
+            typedef short s4_t __attribute__((ext_vector_type(4)));
+            const s4_t a = *reinterpret_cast<const s4_t*>(p), b = *reinterpret_cast<const s4_t*>(p + 4);
+            return s8_t{a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
+        } else {
+            return *reinterpret_cast<const s8_t*>(p);
+        }
+    }
     static __device__ __forceinline__ void compute(acc_t (&acc)[WM][WN], const unsigned short* __restrict__ S,
                                                    int lane, int wr, int wc) {
         const unsigned short* As = S;
@@ -311,29 +326,29 @@ struct GemmNTbf16 {
             s8_t av[WM], bv[WN];
 #pragma unroll
             for (int a = 0; a < WM; a++)
-                av[a] = *reinterpret_cast<const s8_t*>(As + (wr * (WM * 16) + a * 16 + (lane & 15)) * PK + kg);
+                av[a] = ld8(As + (wr * (WM * 16) + a * 16 + (lane & 15)) * PK + kg);
 #pragma unroll
             for (int b = 0; b < WN; b++)
-                bv[b] = *reinterpret_cast<const s8_t*>(Bs + (wc * (WN * 16) + b * 16 + (lane & 15)) * PK + kg);
+                bv[b] = ld8(Bs + (wc * (WN * 16) + b * 16 + (lane & 15)) * PK + kg);
 #pragma unroll
             for (int a = 0; a < WM; a++)
 #pragma unroll
                 for (int b = 0; b < WN; b++)
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = mma(av[a], bv[b], acc[a][b]);
             if constexpr (SPLIT) {
                 s8_t avl[WM], bvl[WN];
 #pragma unroll
                 for (int a = 0; a < WM; a++)
-                    avl[a] = *reinterpret_cast<const s8_t*>(Al + (wr * (WM * 16) + a * 16 + (lane & 15)) * PK + kg);
+                    avl[a] = ld8(Al + (wr * (WM * 16) + a * 16 + (lane & 15)) * PK + kg);
 #pragma unroll
                 for (int b = 0; b < WN; b++)
-                    bvl[b] = *reinterpret_cast<const s8_t*>(Bl + (wc * (WN * 16) + b * 16 + (lane & 15)) * PK + kg);
+                    bvl[b] = ld8(Bl + (wc * (WN * 16) + b * 16 + (lane & 15)) * PK + kg);
 #pragma unroll
                 for (int a = 0; a < WM; a++)
 #pragma unroll
                     for (int b = 0; b < WN; b++) {
-                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bvl[b], acc[a][b], 0, 0, 0);
-                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(avl[a], bv[b], acc[a][b], 0, 0, 0);
+                        acc[a][b] = mma(av[a], bvl[b], acc[a][b]);
+                        acc[a][b] = mma(avl[a], bv[b], acc[a][b]);
                     }
             }
         }
@@ -381,9 +396,11 @@ struct GemmNTbf16 {
     }
 };
 
-// K-chunk of the 128x128 FP64/FP32 GEMM items: 32 (FP64: 8 k-steps of 64-cycle MFMAs per
-// chunk, 130 KB of LDS; FP32: 65 KB, so that two workgroups fit a CU where registers allow).
-template <typename TC> constexpr int gemm_bk() { return 32; }
+// K-chunk of the 128x128 FP64/FP32 GEMM items: 16 FP64 / 32 FP32 columns, 65 KB of LDS for the
+// two buffers, so that two workgroups share a CU: one's barriers, loads and C traffic overlap
+// the other's MFMAs (FP64, m = n = 16384, K = 512: 43.5 TF/s with one 130 KB workgroup per CU,
+// 53 TF/s with two; 66 TF/s with the 16-byte loads, coalesced C and C read up front).
+template <typename TC> constexpr int gemm_bk() { return sizeof(TC) == 8 ? 16 : 32; }
 
 // ---------------------------------------------------------------------------
 // GEMM / SYRK work item: C[r0:r0+128, c0:c0+128] = beta*C + alpha * A[r0:,:] B[c0:,:]^T
@@ -391,49 +408,55 @@ template <typename TC> constexpr int gemm_bk() { return 32; }
 // lower triangle (global row >= global col) is written (dsyrk 'L').
 // beta == 0 overwrites C without reading it (accumulate-mode temporaries).
 // ---------------------------------------------------------------------------
-template <typename TS, int PREC, int DEPTH = 1>
+// CPRE (the Cholesky update, alpha = -1, beta = 1): the accumulators start from -C, read
+// together with the first K-chunk instead of after the last one, and C = -acc is stored.
+template <typename TS, int PREC, bool CPRE = false>
 __device__ __forceinline__ void item_gemm(const TS* __restrict__ A, long lda, const TS* __restrict__ B, long ldb, TS* __restrict__ C,
                           long ldc, int m, int n, int K, int r0, int c0, bool syrk, double alpha, double beta,
                           char* lds) {
     const int mv = min(GEMM_BM, m - r0), nv = min(GEMM_BM, n - c0);
     if (mv <= 0 || nv <= 0) return;
+    auto body = [&](auto* gsel) {
+        using G = typename std::remove_pointer<decltype(gsel)>::type;
+        typedef typename G::acc_t acc_t;
+        acc_t acc[G::WM][G::WN];
+        auto owned = [&](int rr, int cc) { return rr < mv && cc < nv && (!syrk || r0 + rr >= c0 + cc); };
+        if constexpr (CPRE) {
+#pragma unroll
+            for (int a = 0; a < G::WM; a++)
+#pragma unroll
+                for (int b = 0; b < G::WN; b++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int rr = G::row_of(a, r), cc = G::col_of(b, r);
+                        acc[a][b][r] = owned(rr, cc) ? -C[(r0 + rr) + (long)(c0 + cc) * ldc] : TS(0);
+                    }
+        } else {
+            G::zero(acc);
+        }
+        G::run(acc, A + r0, lda, mv, B + c0, ldb, nv, K, lds);
+#pragma unroll
+        for (int a = 0; a < G::WM; a++)
+#pragma unroll
+            for (int b = 0; b < G::WN; b++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int rr = G::row_of(a, r), cc = G::col_of(b, r);
+                    if (owned(rr, cc)) {
+                        TS* p = C + (r0 + rr) + (long)(c0 + cc) * ldc;
+                        if constexpr (CPRE) {
+                            *p = (TS)(-acc[a][b][r]);
+                        } else {
+                            const double v = alpha * (double)acc[a][b][r];
+                            *p = (TS)(beta == 0.0 ? v : beta * (double)(*p) + v);
+                        }
+                    }
+                }
+    };
     if constexpr (PREC == P_BF16 || PREC == P_BF16X3) {
-        using G = GemmNTbf16<TS, GEMM_BM, GEMM_BM, PREC == P_BF16X3>;
-        typename G::acc_t acc[G::WM][G::WN];
-        G::zero(acc);
-        G::run(acc, A + r0, lda, mv, B + c0, ldb, nv, K, lds);
-#pragma unroll
-        for (int a = 0; a < G::WM; a++)
-#pragma unroll
-            for (int b = 0; b < G::WN; b++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int rr = G::row_of(a, r), cc = G::col_of(b);
-                    if (rr < mv && cc < nv && (!syrk || r0 + rr >= c0 + cc)) {
-                        TS* p = C + (r0 + rr) + (long)(c0 + cc) * ldc;
-                        const double v = alpha * (double)acc[a][b][r];
-                        *p = (TS)(beta == 0.0 ? v : beta * (double)(*p) + v);
-                    }
-                }
-    } else {
-        typedef TS TC;  // F64 tiles on FP64 MFMA, F32 tiles on FP32 MFMA
-        using G = GemmNT<TS, TS, TC, GEMM_BM, GEMM_BM, gemm_bk<TC>(), DEPTH>;
-        typename G::acc_t acc[G::WM][G::WN];
-        G::zero(acc);
-        G::run(acc, A + r0, lda, mv, B + c0, ldb, nv, K, lds);
-#pragma unroll
-        for (int a = 0; a < G::WM; a++)
-#pragma unroll
-            for (int b = 0; b < G::WN; b++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int rr = G::row_of(a, r), cc = G::col_of(b);
-                    if (rr < mv && cc < nv && (!syrk || r0 + rr >= c0 + cc)) {
-                        TS* p = C + (r0 + rr) + (long)(c0 + cc) * ldc;
-                        const TC v = (TC)alpha * acc[a][b][r];
-                        *p = (beta == 0.0) ? v : (TC)beta * (*p) + v;
-                    }
-                }
+        body((GemmNTbf16<TS, GEMM_BM, GEMM_BM, PREC == P_BF16X3, true>*)nullptr);
+    } else {  // F64 tiles on FP64 MFMA, F32 tiles on FP32 MFMA
+        body((GemmNT<TS, TS, TS, GEMM_BM, GEMM_BM, gemm_bk<TS>(), true, true>*)nullptr);
     }
 }
 
@@ -599,8 +622,8 @@ __device__ __forceinline__ void lds_trtri64(double* D, double* X, int w) {
 constexpr int POTRF_BM = 128;  // rows per panel GEMM call
 template <typename TS>
 struct PotrfGemm {
-    using G = GemmNT<TS, TS, double, POTRF_BM, DB, 32>;        // panel update
-    using G2 = GemmNT<TS, double, double, POTRF_BM, DB, 32>;   // times the inverse
+    using G = GemmNT<TS, TS, double, POTRF_BM, DB, 16>;        // panel update
+    using G2 = GemmNT<TS, double, double, POTRF_BM, DB, 16>;   // times the inverse
 };
 
 template <typename TS>
@@ -628,7 +651,7 @@ __device__ __forceinline__ void item_potrf(TS* __restrict__ A, long lda, int n, 
                     for (int b = 0; b < G::WN; b++)
 #pragma unroll
                         for (int r = 0; r < 4; r++) {
-                            const int rr = G::row_of(a, r), cc = G::col_of(b);
+                            const int rr = G::row_of(a, r), cc = G::col_of(b, r);
                             if (rr < mv && cc < pw && (r0 + rr >= cc)) {
                                 TS* q = T + rr + (long)cc * lda;
                                 *q = (TS)((double)(*q) - acc[a][b][r]);
@@ -667,7 +690,7 @@ __device__ __forceinline__ void item_potrf(TS* __restrict__ A, long lda, int n, 
                 for (int b = 0; b < G::WN; b++)
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
-                        const int rr = G::row_of(a, r), cc = G::col_of(b);
+                        const int rr = G::row_of(a, r), cc = G::col_of(b, r);
                         if (rr < mv && cc < pw) T[rr + (long)cc * lda] = (TS)acc[a][b][r];
                     }
         }
@@ -703,7 +726,7 @@ __device__ __forceinline__ void item_trsm(const TS* __restrict__ L, long ldl, co
             for (int b = 0; b < G::WN; b++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const int rr = G::row_of(a, r), cc = G::col_of(b);
+                    const int rr = G::row_of(a, r), cc = G::col_of(b, r);
                     if (rr < mv && cc < cw) {
                         TS* p = T + rr + (long)cc * ldx;
                         *p = (TS)((TC)(*p) - acc[a][b][r]);
@@ -718,7 +741,7 @@ __device__ __forceinline__ void item_trsm(const TS* __restrict__ L, long ldl, co
             for (int b = 0; b < G::WN; b++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const int rr = G::row_of(a, r), cc = G::col_of(b);
+                    const int rr = G::row_of(a, r), cc = G::col_of(b, r);
                     if (rr < mv && cc < cw) T[rr + (long)cc * ldx] = (TS)acc[a][b][r];
                 }
         __syncthreads();
@@ -726,10 +749,10 @@ __device__ __forceinline__ void item_trsm(const TS* __restrict__ L, long ldl, co
 }
 
 // LDS bytes each item kind needs (the executor reserves the maximum).
-constexpr int LDS_GEMM_F64 = GemmNT<double, double, double, GEMM_BM, GEMM_BM, gemm_bk<double>()>::LDS_BYTES;
-constexpr int LDS_GEMM_F32 = GemmNT<float, float, float, GEMM_BM, GEMM_BM, gemm_bk<float>()>::LDS_BYTES;
-constexpr int LDS_GEMM_BF16 = GemmNTbf16<float, GEMM_BM, GEMM_BM, true>::LDS_BYTES;
-constexpr int LDS_GEMM_BF16_1 = GemmNTbf16<float, GEMM_BM, GEMM_BM, false>::LDS_BYTES;
+constexpr int LDS_GEMM_F64 = GemmNT<double, double, double, GEMM_BM, GEMM_BM, gemm_bk<double>(), true, true>::LDS_BYTES;
+constexpr int LDS_GEMM_F32 = GemmNT<float, float, float, GEMM_BM, GEMM_BM, gemm_bk<float>(), true, true>::LDS_BYTES;
+constexpr int LDS_GEMM_BF16 = GemmNTbf16<float, GEMM_BM, GEMM_BM, true, true>::LDS_BYTES;
+constexpr int LDS_GEMM_BF16_1 = GemmNTbf16<float, GEMM_BM, GEMM_BM, false, true>::LDS_BYTES;
 // LDS of one GEMM item of the given storage / precision (the standalone kernels launch with it).
 template <typename TS, int PREC> constexpr int gemm_item_lds() {
     return PREC == P_BF16X3 ? LDS_GEMM_BF16 : PREC == P_BF16 ? LDS_GEMM_BF16_1 : sizeof(TS) == 8 ? LDS_GEMM_F64 : LDS_GEMM_F32;
@@ -737,6 +760,7 @@ template <typename TS, int PREC> constexpr int gemm_item_lds() {
 constexpr int LDS_POTRF = PotrfGemm<double>::G::LDS_BYTES > 2 * DB * DP * 8 ? PotrfGemm<double>::G::LDS_BYTES
                                                                              : 2 * DB * DP * 8;
 constexpr int LDS_TRSM = GemmNT<double, double, double, TRSM_BM, DB, 32>::LDS_BYTES;
+constexpr int LDS_TRSM_F32 = GemmNT<float, float, float, TRSM_BM, DB, 32>::LDS_BYTES;
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 constexpr int LDS_ITEM_MAX = cmax(cmax(cmax(LDS_GEMM_F64, LDS_GEMM_F32), cmax(LDS_GEMM_BF16, LDS_POTRF)), LDS_TRSM);
 
