@@ -700,26 +700,31 @@ __device__ __forceinline__ void item_potrf(TS* __restrict__ A, long lda, int n, 
 
 // ---------------------------------------------------------------------------
 // TRSM work item: rows [r0, r0+TRSM_BM) of X (m x n) := X L^{-T} (Right, Lower,
-// Trans, NonUnit), in place, by 64-column blocks: subtract the already solved
-// columns times L's off-diagonal blocks, then multiply by the inverse of L's
-// 64x64 diagonal block left by POTRF. TC is the MFMA type (FP64, or FP32 for the
+// Trans, NonUnit), in place, by 64-column blocks: the block's right-hand side goes
+// into the accumulators, the already solved columns times L's off-diagonal blocks
+// are subtracted (one GEMM), and the result is multiplied by the inverse of L's
+// 64x64 diagonal block left by POTRF straight from LDS (no round trip through
+// memory between the two products). TC is the MFMA type (FP64, or FP32 for the
 // mixed variants).
 // ---------------------------------------------------------------------------
 template <typename TS, typename TC>
 __device__ __forceinline__ void item_trsm(const TS* __restrict__ L, long ldl, const double* __restrict__ dinv, TS* __restrict__ X,
                           long ldx, int m, int n, int r0, char* lds) {
-    using G = GemmNT<TS, TS, TC, TRSM_BM, DB, 32>;
-    using G2 = GemmNT<TS, double, TC, TRSM_BM, DB, 32>;
+    constexpr bool VEC = std::is_same<TS, TC>::value;
+    using G = GemmNT<TS, TS, TC, TRSM_BM, DB, 32, VEC, true>;     // update from the solved columns
+    using G3 = GemmNT<TC, TC, TC, TRSM_BM, DB, DB, false, true>;  // times Dinv^T, both operands in LDS
     const int mv = min(TRSM_BM, m - r0);
     if (mv <= 0) return;
     TS* Xr = X + r0;
     const int nblk = (n + DB - 1) / DB;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1;
+    TC* At = reinterpret_cast<TC*>(lds);  // G3 staging: At[k][m] = t(m, k), Bt[k][j] = Dinv(j, k)
+    TC* Bt = At + DB * G3::PA;
     typename G::acc_t acc[G::WM][G::WN];
     for (int cb = 0; cb < nblk; cb++) {
         const int c0 = cb * DB, cw = min(DB, n - c0);
         TS* T = Xr + (long)c0 * ldx;
-        G::zero(acc);
-        if (c0 > 0) G::run(acc, Xr, ldx, mv, L + c0, ldl, cw, c0, lds);
+        // acc = -(right-hand side block) + X[:, :c0] L[c0:c0+cw, :c0]^T = -t
 #pragma unroll
         for (int a = 0; a < G::WM; a++)
 #pragma unroll
@@ -727,27 +732,35 @@ __device__ __forceinline__ void item_trsm(const TS* __restrict__ L, long ldl, co
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int rr = G::row_of(a, r), cc = G::col_of(b, r);
-                    if (rr < mv && cc < cw) {
-                        TS* p = T + rr + (long)cc * ldx;
-                        *p = (TS)((TC)(*p) - acc[a][b][r]);
-                    }
+                    acc[a][b][r] = (rr < mv && cc < cw) ? -(TC)T[rr + (long)cc * ldx] : TC(0);
                 }
+        if (c0 > 0) G::run(acc, Xr, ldx, mv, L + c0, ldl, cw, c0, lds);
+        else __syncthreads();  // the previous block's products have left the staging area
+#pragma unroll
+        for (int a = 0; a < G::WM; a++)
+#pragma unroll
+            for (int b = 0; b < G::WN; b++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) At[G::col_of(b, r) * G3::PA + G::row_of(a, r)] = -acc[a][b][r];
+        const double* Dv = dinv + (long)cb * DB * DB;  // Dv[i + j*DB] = Dinv(i, j)
+        for (int e = threadIdx.x; e < DB * DB; e += NTHREADS) Bt[(e / DB) * G3::PB + (e % DB)] = (TC)Dv[e];
         __syncthreads();
-        G2::zero(acc);
-        G2::run(acc, T, ldx, mv, dinv + (long)cb * DB * DB, DB, cw, cw, lds);
+        G3::zero(acc);
+        G3::compute(acc, At, lane, wr, wc);
 #pragma unroll
         for (int a = 0; a < G::WM; a++)
 #pragma unroll
             for (int b = 0; b < G::WN; b++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const int rr = G::row_of(a, r), cc = G::col_of(b, r);
+                    const int rr = G3::row_of(a, r), cc = G3::col_of(b, r);
                     if (rr < mv && cc < cw) T[rr + (long)cc * ldx] = (TS)acc[a][b][r];
                 }
         __syncthreads();
     }
 }
 
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
 // LDS bytes each item kind needs (the executor reserves the maximum).
 constexpr int LDS_GEMM_F64 = GemmNT<double, double, double, GEMM_BM, GEMM_BM, gemm_bk<double>(), true, true>::LDS_BYTES;
 constexpr int LDS_GEMM_F32 = GemmNT<float, float, float, GEMM_BM, GEMM_BM, gemm_bk<float>(), true, true>::LDS_BYTES;
@@ -759,9 +772,10 @@ template <typename TS, int PREC> constexpr int gemm_item_lds() {
 }
 constexpr int LDS_POTRF = PotrfGemm<double>::G::LDS_BYTES > 2 * DB * DP * 8 ? PotrfGemm<double>::G::LDS_BYTES
                                                                              : 2 * DB * DP * 8;
-constexpr int LDS_TRSM = GemmNT<double, double, double, TRSM_BM, DB, 32>::LDS_BYTES;
-constexpr int LDS_TRSM_F32 = GemmNT<float, float, float, TRSM_BM, DB, 32>::LDS_BYTES;
-constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int LDS_TRSM = cmax(GemmNT<double, double, double, TRSM_BM, DB, 32, true, true>::LDS_BYTES,
+                              GemmNT<double, double, double, TRSM_BM, DB, DB, false, true>::STAGE * 8);
+constexpr int LDS_TRSM_F32 = cmax(GemmNT<float, float, float, TRSM_BM, DB, 32, true, true>::LDS_BYTES,
+                                  GemmNT<float, float, float, TRSM_BM, DB, DB, false, true>::STAGE * 4);
 constexpr int LDS_ITEM_MAX = cmax(cmax(cmax(LDS_GEMM_F64, LDS_GEMM_F32), cmax(LDS_GEMM_BF16, LDS_POTRF)), LDS_TRSM);
 
 }  // namespace ttad
