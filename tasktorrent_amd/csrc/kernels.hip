@@ -54,8 +54,9 @@ __device__ __forceinline__ void push_task(const DevGraph& g, int s) {
         __hip_atomic_store(g.abort_flag, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
+    // the caller's release fence orders these entries after the outputs they announce
     for (int r = 0; r < t.n_items; r++)
-        __hip_atomic_store(g.q_items + g.q_base[b] + pos + r, ((int64_t)s << 20) | r, __ATOMIC_RELEASE,
+        __hip_atomic_store(g.q_items + g.q_base[b] + pos + r, ((int64_t)s << 20) | r, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -182,14 +183,23 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
                     rec[3] = wall_clock64();
                 }
             }
+            // Counters are relaxed: the release fence above publishes this item's outputs (one L2 write-back
+            // per item; every release at agent scope writes back all dirty lines of the XCD's L2). The thread
+            // that takes a counter to zero acquires what the other decrements released and releases it again
+            // before its own next counter update or queue entries (one acq_rel fence per task, and per
+            // successor it makes ready).
             const int left =
-                __hip_atomic_fetch_add(g.items_left + task, -1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) - 1;
+                __hip_atomic_fetch_add(g.items_left + task, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - 1;
             if (left == 0) {  // last item of the task: release its successors
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
                 for (int q = g.out_ptr[task]; q < g.out_ptr[task + 1]; q++) {
                     const int s = g.out_idx[q];
                     const int d =
-                        __hip_atomic_fetch_add(g.deps + s, -1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) - 1;
-                    if (d == 0) push_task(g, s);
+                        __hip_atomic_fetch_add(g.deps + s, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - 1;
+                    if (d == 0) {
+                        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+                        push_task(g, s);
+                    }
                 }
                 __hip_atomic_fetch_add(g.done_items, t.n_items, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
