@@ -66,7 +66,7 @@ __device__ __forceinline__ void load_a(TL (&a)[KS], const DevMat& M, const Sweep
     const bool mok = m < wp;
     if (mode == 0) {
         const long ld = M.bsz[tp];
-        const TL* T = (const TL*)M.tiles[tp + (long)tq * M.nt] + (op + m) + (long)oq * ld;
+        const TTAD_GLOBAL TL* T = gmem((const TL*)M.tiles[tp + (long)tq * M.nt]) + (op + m) + (long)oq * ld;
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
             const int k = 4 * ks + kg;
@@ -74,7 +74,7 @@ __device__ __forceinline__ void load_a(TL (&a)[KS], const DevMat& M, const Sweep
         }
     } else if (mode == 1) {
         const long ld = M.bsz[tq];
-        const TL* T = (const TL*)M.tiles[tq + (long)tp * M.nt] + oq + (long)(op + m) * ld;
+        const TTAD_GLOBAL TL* T = gmem((const TL*)M.tiles[tq + (long)tp * M.nt]) + oq + (long)(op + m) * ld;
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
             const int k = 4 * ks + kg;
@@ -82,7 +82,7 @@ __device__ __forceinline__ void load_a(TL (&a)[KS], const DevMat& M, const Sweep
         }
     } else {
         const long ld = M.bsz[tp];
-        const TL* T = (const TL*)M.tiles[tp + (long)tp * M.nt] + op + (long)op * ld;
+        const TTAD_GLOBAL TL* T = gmem((const TL*)M.tiles[tp + (long)tp * M.nt]) + op + (long)op * ld;
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
             const int k = 4 * ks + kg;
@@ -98,7 +98,7 @@ __device__ __forceinline__ void load_b(TC (&b)[NR / 16][KS], const TS* Y, long l
 #pragma unroll
     for (int nb = 0; nb < NR / 16; nb++) {
         const int n = nb * 16 + (lane & 15);
-        const TS* Yc = Y + g0 + (long)n * ldy;
+        const TTAD_GLOBAL TS* Yc = gmem(Y) + g0 + (long)n * ldy;
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
             const int k = 4 * ks + kg;
@@ -130,7 +130,7 @@ __device__ __forceinline__ void fill_block(TC* Ls, const DevMat& M, const SweepT
     const int nt = M.nt;
     constexpr int LD = SB + SPAD;
     if (mode == 0) {
-        const TL* T = (const TL*)M.tiles[tp + (long)tq * nt];
+        const TTAD_GLOBAL TL* T = gmem((const TL*)M.tiles[tp + (long)tq * nt]);
         const long ld = M.bsz[tp];
         for (int e = threadIdx.x; e < SB * SB; e += NTHREADS) {
             const int m = e % SB, k = e / SB;
@@ -138,7 +138,7 @@ __device__ __forceinline__ void fill_block(TC* Ls, const DevMat& M, const SweepT
             Ls[k * LD + m] = absval ? (TC)fabs((double)v) : v;
         }
     } else if (mode == 1) {
-        const TL* T = (const TL*)M.tiles[tq + (long)tp * nt];
+        const TTAD_GLOBAL TL* T = gmem((const TL*)M.tiles[tq + (long)tp * nt]);
         const long ld = M.bsz[tq];
         for (int e = threadIdx.x; e < SB * SB; e += NTHREADS) {
             const int k = e % SB, m = e / SB;  // contiguous in k
@@ -146,7 +146,7 @@ __device__ __forceinline__ void fill_block(TC* Ls, const DevMat& M, const SweepT
             Ls[k * LD + m] = absval ? (TC)fabs((double)v) : v;
         }
     } else {
-        const TL* T = (const TL*)M.tiles[tp + (long)tp * nt];
+        const TTAD_GLOBAL TL* T = gmem((const TL*)M.tiles[tp + (long)tp * nt]);
         const long ld = M.bsz[tp];
         for (int e = threadIdx.x; e < SB * SB; e += NTHREADS) {
             const int m = e % SB, k = e / SB;
@@ -163,7 +163,7 @@ __device__ __forceinline__ void fill_rhs(TC* Ys, const TS* Y, long ldy, int g0, 
     constexpr int LD = NR + SPAD;
     for (int e = threadIdx.x; e < SB * NR; e += NTHREADS) {
         const int k = e % SB, n = e / SB;
-        Ys[k * LD + n] = (k < w && n < nrhs) ? (TC)Y[(g0 + k) + (long)n * ldy] : TC(0);
+        Ys[k * LD + n] = (k < w && n < nrhs) ? (TC)gmem(Y)[(g0 + k) + (long)n * ldy] : TC(0);
     }
 }
 
@@ -243,10 +243,10 @@ __device__ __forceinline__ void item_sweep(const DevMat& M, const SweepTab& tab,
             const int m = wv * 16 + Mfma<TC>::row(lane, r), n = nb * 16 + (lane & 15);
             rhs[nb][r] = TC(0);
             if (m < w && n < nrhs)
-                rhs[nb][r] = forward ? (TC)R[(g0 + m) + (long)n * ldr] : Y[(g0 + m) + (long)n * ldy];
+                rhs[nb][r] = forward ? (TC)gmem(R)[(g0 + m) + (long)n * ldr] : gmem(Y)[(g0 + m) + (long)n * ldy];
         }
     {
-        const double* Dv = M.dinv[tab.blk_tile[p]] + (long)(tab.blk_off[p] / SB) * SB * SB;
+        const TTAD_GLOBAL double* Dv = gmem(M.dinv[tab.blk_tile[p]]) + (long)(tab.blk_off[p] / SB) * SB * SB;
         for (int e = threadIdx.x; e < SB * SB; e += NTHREADS) {
             const int a = e % SB, b = e / SB;  // Dv[a + b*64] = Dinv(a, b)
             if (forward) Ds[b * LD + a] = (TC)Dv[e];   // (m, k) = Dinv(m, k)
@@ -334,10 +334,10 @@ __device__ __forceinline__ void item_sweep(const DevMat& M, const SweepTab& tab,
             if (m < w && n < nrhs) {
                 const long gi = g0 + m;
                 if (forward) {
-                    Y[gi + (long)n * ldy] = acc[nb][r];
+                    gmem(Y)[gi + (long)n * ldy] = acc[nb][r];
                 } else {
-                    D[gi + (long)n * ldd] = acc[nb][r];
-                    X[gi + (long)n * ldx] += (double)acc[nb][r];
+                    gmem(D)[gi + (long)n * ldd] = acc[nb][r];
+                    gmem(X)[gi + (long)n * ldx] += (double)acc[nb][r];
                 }
             }
         }
@@ -446,11 +446,11 @@ __device__ __forceinline__ void item_resid(const DevMat& M, const SweepTab& tab,
                 if (n == 0) atomicMax(norms, (unsigned long long)__double_as_longlong(acc[nb][r]));
                 continue;
             }
-            const double b = B[gi + (long)n * ldb];
+            const double b = gmem(B)[gi + (long)n * ldb];
             const double rv = b - acc[nb][r];
-            R[gi + (long)n * ldr] = rv;
+            gmem(R)[gi + (long)n * ldr] = rv;
             atomicMax(norms + 3 * n + 0, (unsigned long long)__double_as_longlong(fabs(rv)));
-            atomicMax(norms + 3 * n + 1, (unsigned long long)__double_as_longlong(fabs(X[gi + (long)n * ldx])));
+            atomicMax(norms + 3 * n + 1, (unsigned long long)__double_as_longlong(fabs(gmem(X)[gi + (long)n * ldx])));
             atomicMax(norms + 3 * n + 2, (unsigned long long)__double_as_longlong(fabs(b)));
         }
 }

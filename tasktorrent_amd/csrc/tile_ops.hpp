@@ -18,6 +18,16 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 typedef short s8_t __attribute__((ext_vector_type(8)));
 
+// Device memory accesses of the item code go through address-space-1 pointers. Tile
+// pointers reach it through tables in memory, so the compiler sees generic pointers and
+// emits flat instructions, which count against the LDS wait counter as well: every LDS
+// wait of a GEMM loop would then also wait for the prefetched loads of the next chunk.
+#define TTAD_GLOBAL __attribute__((address_space(1)))
+template <typename T> __device__ __forceinline__ const TTAD_GLOBAL T* gmem(const T* p) {
+    return (const TTAD_GLOBAL T*)p;
+}
+template <typename T> __device__ __forceinline__ TTAD_GLOBAL T* gmem(T* p) { return (TTAD_GLOBAL T*)p; }
+
 // ---------------------------------------------------------------------------
 // MFMA 16x16 primitives. Operand layouts (lane l of the wave):
 //   f64/f32 16x16x4 : A[m = l%16][k = l/16], B[k = l/16][n = l%16]
@@ -120,16 +130,16 @@ struct GemmNT {
         if (nvalid >= L * NV && k0 + KS * E <= K) {  // whole chunk inside (uniform): no guards
 #pragma unroll
             for (int s = 0; s < E; s++) {
-                const T* p = base + row + (long)(k0 + kc + s * KS) * ld;
+                const TTAD_GLOBAL T* p = gmem(base) + row + (long)(k0 + kc + s * KS) * ld;
                 if constexpr (NV == 1) reg[s] = *p;
-                else reg[s] = *reinterpret_cast<const typename V16<T>::u*>(p);
+                else reg[s] = *reinterpret_cast<const TTAD_GLOBAL typename V16<T>::u*>(p);
             }
             return;
         }
 #pragma unroll
         for (int s = 0; s < E; s++) {
             const int kk = k0 + kc + s * KS;
-            const T* p = base + row + (long)kk * ld;
+            const TTAD_GLOBAL T* p = gmem(base) + row + (long)kk * ld;
             if constexpr (NV == 1) {
                 reg[s] = (row < nvalid && kk < K) ? *p : T(0);
             } else {
@@ -262,14 +272,26 @@ struct GemmNTbf16 {
                     : __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
     }
 
+    // full: the whole chunk is inside the operand (uniform), so the loads need no guards; the
+    // guarded form, one exec-masked branch per load, is kept for edge chunks.
     template <int G, int GS>
     static __device__ __forceinline__ void load_raw(float (&r)[G][8], const TA* __restrict__ base, long ld,
-                                                    bool row_ok, int kfirst, int K) {
+                                                    bool row_ok, int kfirst, int K, bool full) {
+        const TTAD_GLOBAL TA* gb = gmem(base);
+        if (full) {
+#pragma unroll
+            for (int s = 0; s < G; s++) {
+                const int k = kfirst + s * GS * 8;
+#pragma unroll
+                for (int q = 0; q < 8; q++) r[s][q] = (float)gb[(long)(k + q) * ld];
+            }
+            return;
+        }
 #pragma unroll
         for (int s = 0; s < G; s++) {
             const int k = kfirst + s * GS * 8;
 #pragma unroll
-            for (int q = 0; q < 8; q++) r[s][q] = (row_ok && k + q < K) ? (float)base[(long)(k + q) * ld] : 0.0f;
+            for (int q = 0; q < 8; q++) r[s][q] = (row_ok && k + q < K) ? (float)gb[(long)(k + q) * ld] : 0.0f;
         }
     }
 
@@ -373,12 +395,15 @@ struct GemmNTbf16 {
             store_planes<GA, NTHREADS / BM>(S, S + (BM + BN) * PK, ra, a_row, a_g);
             store_planes<GB, NTHREADS / BN>(S + BM * PK, S + (BM + BN) * PK + BM * PK, rb, b_row, b_g);
         };
-        load_raw<GA, NTHREADS / BM>(ra, Ab, lda, a_ok, a_g * 8, K);
-        load_raw<GB, NTHREADS / BN>(rb, Bb, ldb, b_ok, b_g * 8, K);
+        // a chunk is full when both operands have all their rows and the chunk lies inside K
+        const bool rows_full = mv >= BM && nv >= BN;
+        auto full = [&](int k0) { return rows_full && k0 + BK <= K; };
+        load_raw<GA, NTHREADS / BM>(ra, Ab, lda, a_ok, a_g * 8, K, full(0));
+        load_raw<GB, NTHREADS / BN>(rb, Bb, ldb, b_ok, b_g * 8, K, full(0));
         stage(S0);
         if (nchunks > 1) {
-            load_raw<GA, NTHREADS / BM>(ra, Ab, lda, a_ok, BK + a_g * 8, K);
-            load_raw<GB, NTHREADS / BN>(rb, Bb, ldb, b_ok, BK + b_g * 8, K);
+            load_raw<GA, NTHREADS / BM>(ra, Ab, lda, a_ok, BK + a_g * 8, K, full(BK));
+            load_raw<GB, NTHREADS / BN>(rb, Bb, ldb, b_ok, BK + b_g * 8, K, full(BK));
         }
         __syncthreads();
         for (int c = 0; c < nchunks; c++) {
@@ -387,8 +412,8 @@ struct GemmNTbf16 {
                 stage(S0 + ((c + 1) & 1) * STAGE);
                 if (c + 2 < nchunks) {
                     const int k0 = (c + 2) * BK;
-                    load_raw<GA, NTHREADS / BM>(ra, Ab, lda, a_ok, k0 + a_g * 8, K);
-                    load_raw<GB, NTHREADS / BN>(rb, Bb, ldb, b_ok, k0 + b_g * 8, K);
+                    load_raw<GA, NTHREADS / BM>(ra, Ab, lda, a_ok, k0 + a_g * 8, K, full(k0));
+                    load_raw<GB, NTHREADS / BN>(rb, Bb, ldb, b_ok, k0 + b_g * 8, K, full(k0));
                 }
             }
             __syncthreads();
@@ -429,7 +454,7 @@ __device__ __forceinline__ void item_gemm(const TS* __restrict__ A, long lda, co
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
                         const int rr = G::row_of(a, r), cc = G::col_of(b, r);
-                        acc[a][b][r] = owned(rr, cc) ? -C[(r0 + rr) + (long)(c0 + cc) * ldc] : TS(0);
+                        acc[a][b][r] = owned(rr, cc) ? -gmem(C)[(r0 + rr) + (long)(c0 + cc) * ldc] : TS(0);
                     }
         } else {
             G::zero(acc);
@@ -443,7 +468,7 @@ __device__ __forceinline__ void item_gemm(const TS* __restrict__ A, long lda, co
                 for (int r = 0; r < 4; r++) {
                     const int rr = G::row_of(a, r), cc = G::col_of(b, r);
                     if (owned(rr, cc)) {
-                        TS* p = C + (r0 + rr) + (long)(c0 + cc) * ldc;
+                        TTAD_GLOBAL TS* p = gmem(C) + (r0 + rr) + (long)(c0 + cc) * ldc;
                         if constexpr (CPRE) {
                             *p = (TS)(-acc[a][b][r]);
                         } else {
@@ -642,7 +667,7 @@ __device__ __forceinline__ void item_potrf(TS* __restrict__ A, long lda, int n, 
         if (p0 > 0) {
             for (int r0 = 0; r0 < rows; r0 += POTRF_BM) {
                 const int mv = min(POTRF_BM, rows - r0);
-                TS* T = A + (p0 + r0) + (long)p0 * lda;
+                TTAD_GLOBAL TS* T = gmem(A) + (p0 + r0) + (long)p0 * lda;
                 G::zero(acc);
                 G::run(acc, A + p0 + r0, lda, mv, A + p0, lda, pw, p0, lds);
 #pragma unroll
@@ -653,7 +678,7 @@ __device__ __forceinline__ void item_potrf(TS* __restrict__ A, long lda, int n, 
                         for (int r = 0; r < 4; r++) {
                             const int rr = G::row_of(a, r), cc = G::col_of(b, r);
                             if (rr < mv && cc < pw && (r0 + rr >= cc)) {
-                                TS* q = T + rr + (long)cc * lda;
+                                TTAD_GLOBAL TS* q = T + rr + (long)cc * lda;
                                 *q = (TS)((double)(*q) - acc[a][b][r]);
                             }
                         }
@@ -664,7 +689,7 @@ __device__ __forceinline__ void item_potrf(TS* __restrict__ A, long lda, int n, 
         for (int e = tid; e < DB * DB; e += NTHREADS) {
             const int i = e % DB, j = e / DB;
             double v;
-            if (i < pw && j < pw) v = (i >= j) ? (double)A[(p0 + i) + (long)(p0 + j) * lda] : 0.0;
+            if (i < pw && j < pw) v = (i >= j) ? (double)gmem(A)[(p0 + i) + (long)(p0 + j) * lda] : 0.0;
             else v = (i == j) ? 1.0 : 0.0;
             D[i + j * DP] = v;
         }
@@ -674,14 +699,15 @@ __device__ __forceinline__ void item_potrf(TS* __restrict__ A, long lda, int n, 
         double* Xinv = dinv + (long)pb * DB * DB;
         for (int e = tid; e < DB * DB; e += NTHREADS) {
             const int i = e % DB, j = e / DB;
-            if (i < pw && j < pw && i >= j) A[(p0 + i) + (long)(p0 + j) * lda] = (TS)D[i + j * DP];
-            Xinv[e] = X[i + j * DP];
+            if (i < pw && j < pw && i >= j) gmem(A)[(p0 + i) + (long)(p0 + j) * lda] = (TS)D[i + j * DP];
+            gmem(Xinv)[e] = X[i + j * DP];
         }
         __syncthreads();
         // rows below: L(q, p) = A(q, p) Xinv^T
         for (int r0 = DB; r0 < rows; r0 += POTRF_BM) {
             const int mv = min(POTRF_BM, rows - r0);
             TS* T = A + (p0 + r0) + (long)p0 * lda;
+            TTAD_GLOBAL TS* Tg = gmem(T);
             G2::zero(acc);
             G2::run(acc, T, lda, mv, Xinv, DB, pw, pw, lds);
 #pragma unroll
@@ -691,7 +717,7 @@ __device__ __forceinline__ void item_potrf(TS* __restrict__ A, long lda, int n, 
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
                         const int rr = G::row_of(a, r), cc = G::col_of(b, r);
-                        if (rr < mv && cc < pw) T[rr + (long)cc * lda] = (TS)acc[a][b][r];
+                        if (rr < mv && cc < pw) Tg[rr + (long)cc * lda] = (TS)acc[a][b][r];
                     }
         }
         __syncthreads();
@@ -723,7 +749,7 @@ __device__ __forceinline__ void item_trsm(const TS* __restrict__ L, long ldl, co
     typename G::acc_t acc[G::WM][G::WN];
     for (int cb = 0; cb < nblk; cb++) {
         const int c0 = cb * DB, cw = min(DB, n - c0);
-        TS* T = Xr + (long)c0 * ldx;
+        TTAD_GLOBAL TS* T = gmem(Xr) + (long)c0 * ldx;
         // acc = -(right-hand side block) + X[:, :c0] L[c0:c0+cw, :c0]^T = -t
 #pragma unroll
         for (int a = 0; a < G::WM; a++)
@@ -743,7 +769,7 @@ __device__ __forceinline__ void item_trsm(const TS* __restrict__ L, long ldl, co
 #pragma unroll
                 for (int r = 0; r < 4; r++) At[G::col_of(b, r) * G3::PA + G::row_of(a, r)] = -acc[a][b][r];
         const double* Dv = dinv + (long)cb * DB * DB;  // Dv[i + j*DB] = Dinv(i, j)
-        for (int e = threadIdx.x; e < DB * DB; e += NTHREADS) Bt[(e / DB) * G3::PB + (e % DB)] = (TC)Dv[e];
+        for (int e = threadIdx.x; e < DB * DB; e += NTHREADS) Bt[(e / DB) * G3::PB + (e % DB)] = (TC)gmem(Dv)[e];
         __syncthreads();
         G3::zero(acc);
         G3::compute(acc, At, lane, wr, wc);
