@@ -647,6 +647,7 @@ static int graph_prepare(ttad_graph* g) {
         TTAD_HIP(g->d_done.alloc(4));
         TTAD_HIP(g->d_abort.alloc(4));
         TTAD_HIP(g->d_trace_cursor.alloc(4));
+        TTAD_HIP(g->d_cu_slots.alloc((CU_KEYS + 1) * 4));  // + the idle-primary counter
         TTAD_HIP(g->d_mats.alloc(dm.size() * sizeof(DevMat)));
         TTAD_HIP(hipMemcpy(g->d_tasks.p, g->tasks.data(), n * sizeof(DevTask), hipMemcpyHostToDevice));
         TTAD_HIP(hipMemcpy(g->d_out_ptr.p, g->out_ptr.data(), (n + 1) * 4, hipMemcpyHostToDevice));
@@ -713,6 +714,8 @@ static DevGraph graph_image(ttad_graph* g, bool trace, int64_t spin_limit) {
     d.mats = g->d_mats.as<DevMat>();
     d.trace = trace ? g->d_trace.as<uint64_t>() : nullptr;
     d.trace_cursor = g->d_trace_cursor.as<int32_t>();
+    d.cu_slots = g->d_cu_slots.as<int32_t>();
+    d.idle_primary = d.cu_slots + CU_KEYS;
     d.spin_limit = spin_limit;
     return d;
 }

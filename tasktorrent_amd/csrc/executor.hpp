@@ -31,6 +31,8 @@ struct DevGraph {
     const DevMat* mats;
     uint64_t* trace;            // optional: 4 x u64 per item {task<<20|item, wg, t_start, t_end}
     int32_t* trace_cursor;
+    int32_t* cu_slots;          // [CU_KEYS] workers started per CU (slot 0 = primary)
+    int32_t* idle_primary;      // primary workers currently looking for work
     int64_t spin_limit;         // watchdog in wall_clock64 ticks
 };
 
@@ -41,6 +43,16 @@ __device__ __forceinline__ int xcc_id() {
     unsigned v;
     asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
     return (int)(v & 0xF) % NXCD;
+}
+
+// Identifier of the CU this wave runs on: XCC, shader engine, shader array and CU fields of
+// the hardware id registers (fewer than CU_KEYS values).
+constexpr int CU_KEYS = 2048;
+__device__ __forceinline__ int cu_key() {
+    unsigned hw;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_HW_ID)" : "=s"(hw));
+    const int cu = (hw >> 8) & 15, sh = (hw >> 12) & 1, se = (hw >> 13) & 7;
+    return ((xcc_id() * 8 + se) * 2 + sh) * 16 + cu;
 }
 
 __device__ __forceinline__ int ld_relaxed(const int32_t* p) {

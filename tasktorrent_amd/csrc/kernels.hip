@@ -67,6 +67,15 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
     const int tid = threadIdx.x;
     const int margin = 2 * (int)gridDim.x;  // more queued items than this: a fetch-add claim cannot overshoot
     const int my_xcd = xcc_id();
+    // The first worker started on a CU is its primary. A secondary (the second workgroup on that
+    // CU) only takes items from a queue that holds more than there are idle primaries, so that
+    // scarce work spreads one item per CU instead of pairing up on some CUs while others idle.
+    __shared__ int s_primary;
+    if (tid == 0) s_primary = __hip_atomic_fetch_add(g.cu_slots + cu_key(), 1, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT) == 0;
+    __syncthreads();
+    const bool primary = s_primary != 0;
+    if (primary && tid == 0) __hip_atomic_fetch_add(g.idle_primary, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t rng = blockIdx.x * 2654435761u + 0x9e3779b9u;
     for (;;) {
         if (tid < 64) {
@@ -77,6 +86,8 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
                 int fb = -1, fh = 0, ft = 0;  // queue to claim from
                 int sb = -1, sh = 0, st = 0;  // first non-empty queue of any XCD (to steal from)
                 bool any_open = false;
+                // a secondary leaves queues that the idle primaries can drain
+                const int reserve = primary ? 0 : ld_relaxed(g.idle_primary);
                 for (int base = 0; base < g.n_buckets; base += 64) {  // groups of 64 queues, best first
                     const int b = base + tid;
                     int h = 0, t = 0;
@@ -84,7 +95,7 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
                     if (b < g.n_buckets) {
                         h = ld_relaxed(g.q_head + b);
                         t = ld_relaxed(g.q_tail + b);
-                        ne = h < t;
+                        ne = t - h > reserve;
                         open = h < g.q_base[b + 1] - g.q_base[b];
                         const int home = g.q_home[b];
                         mine = ne && (home < 0 || home == my_xcd);
@@ -158,7 +169,11 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
                 }
                 __builtin_amdgcn_s_sleep(2);
             }
-            if (tid == 0) s_item = got;
+            if (tid == 0) {
+                s_item = got;
+                if (primary && got >= 0)
+                    __hip_atomic_fetch_add(g.idle_primary, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
         __syncthreads();
         const int64_t e = s_item;
@@ -203,6 +218,7 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
                 }
                 __hip_atomic_fetch_add(g.done_items, t.n_items, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
+            if (primary) __hip_atomic_fetch_add(g.idle_primary, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
@@ -219,9 +235,11 @@ __global__ void k_graph_reset_a(DevGraph g, int n_slots) {
         g.q_head[x] = 0;
         g.q_tail[x] = g.tail0[x];
     }
+    for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < CU_KEYS; x += stride) g.cu_slots[x] = 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *g.done_items = 0;
         *g.abort_flag = 0;
+        *g.idle_primary = 0;
         if (g.trace_cursor) *g.trace_cursor = 0;
     }
 }
