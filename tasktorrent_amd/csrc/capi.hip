@@ -64,6 +64,7 @@ static std::shared_ptr<StreamPlan> build_stream_plan(ttad_matrix* M, int prec, i
         t.k = k;
         t.n_items = std::max(1, items_of(kind, M->bsz[i], M->bsz[j]));
         t.bucket = 0;
+        t.crit = 0;
         return t;
     };
     for (int k = 0; k < nt; k++) {
@@ -495,6 +496,50 @@ int ttad_last_elapsed(ttad_context* ctx, double* seconds) {
 // ---------------------------------------------------------------------------
 // Device task graphs
 // ---------------------------------------------------------------------------
+// Marks the tasks on a longest path of the DAG, weighting each kind by the latency of its items
+// (POTRF 8, TRSM 3, GEMM/SYRK 1: one workgroup factors a whole tile, a TRSM item solves 64 rows
+// block column by block column). Those items run with no other item started on their CU.
+// TTAD_CRITICAL=0 turns it off (for comparisons).
+static void mark_critical(ttad_graph& g) {
+    static const bool on = [] {
+        const char* v = getenv("TTAD_CRITICAL");
+        return !(v && v[0] == '0');
+    }();
+    const int n = g.n_tasks;
+    for (auto& t : g.tasks) t.crit = 0;
+    if (!on || n == 0) return;
+    std::vector<int> w(n), indeg(n, 0), order;
+    order.reserve(n);
+    for (int t = 0; t < n; t++) {
+        const int kd = g.tasks[t].kind;
+        w[t] = kd == K_POTRF ? 8 : kd == K_TRSM ? 3 : 1;
+    }
+    for (int v : g.out_idx) indeg[v]++;
+    for (int t = 0; t < n; t++)
+        if (indeg[t] == 0) order.push_back(t);
+    for (size_t q = 0; q < order.size(); q++) {
+        const int t = order[q];
+        for (int e = g.out_ptr[t]; e < g.out_ptr[t + 1]; e++)
+            if (--indeg[g.out_idx[e]] == 0) order.push_back(g.out_idx[e]);
+    }
+    if ((int)order.size() != n) return;  // not a DAG: no marking (the graph would not complete anyway)
+    std::vector<long long> top(n, 0), bot(n, 0);
+    for (int t : order) {
+        top[t] += w[t];
+        for (int e = g.out_ptr[t]; e < g.out_ptr[t + 1]; e++)
+            top[g.out_idx[e]] = std::max(top[g.out_idx[e]], top[t]);
+    }
+    long long len = 0;
+    for (int q = n - 1; q >= 0; q--) {
+        const int t = order[q];
+        long long b = 0;
+        for (int e = g.out_ptr[t]; e < g.out_ptr[t + 1]; e++) b = std::max(b, bot[g.out_idx[e]]);
+        bot[t] = b + w[t];
+        len = std::max(len, top[t]);
+    }
+    for (int t = 0; t < n; t++) g.tasks[t].crit = top[t] + bot[t] - w[t] == len;
+}
+
 // GEMM/SYRK work queues homed per XCD (default); TTAD_XCD_AFFINITY=0 turns them into one
 // shared queue per priority bucket (for comparisons).
 static bool xcd_affinity() {
@@ -535,10 +580,12 @@ int ttad_graph_create(ttad_context* ctx, int n_tasks, const ttad_task_desc* task
         x.k = d.k;
         x.n_items = 1;
         x.bucket = 0;
+        x.crit = 0;
     }
     for (int e = 0; e < n_edges; e++)
         if (g->out_idx[e] < 0 || g->out_idx[e] >= n_tasks)
             return set_error(TTAD_ERR_INVALID, "graph_create: edge target %d out of range", g->out_idx[e]);
+    mark_critical(*g);
     // priority -> bucket: distinct priorities ranked highest first, quantized to at most 64
     // buckets (fewer buckets hold more items each: fewer contended claims, measured faster than 256)
     std::vector<double> pr(n_tasks, 0.0);
@@ -647,7 +694,7 @@ static int graph_prepare(ttad_graph* g) {
         TTAD_HIP(g->d_done.alloc(4));
         TTAD_HIP(g->d_abort.alloc(4));
         TTAD_HIP(g->d_trace_cursor.alloc(4));
-        TTAD_HIP(g->d_cu_slots.alloc((CU_KEYS + 1) * 4));  // + the idle-primary counter
+        TTAD_HIP(g->d_cu_slots.alloc((2 * CU_KEYS + 1) * 4));  // slots, critical counts, idle primaries
         TTAD_HIP(g->d_mats.alloc(dm.size() * sizeof(DevMat)));
         TTAD_HIP(hipMemcpy(g->d_tasks.p, g->tasks.data(), n * sizeof(DevTask), hipMemcpyHostToDevice));
         TTAD_HIP(hipMemcpy(g->d_out_ptr.p, g->out_ptr.data(), (n + 1) * 4, hipMemcpyHostToDevice));
@@ -715,7 +762,8 @@ static DevGraph graph_image(ttad_graph* g, bool trace, int64_t spin_limit) {
     d.trace = trace ? g->d_trace.as<uint64_t>() : nullptr;
     d.trace_cursor = g->d_trace_cursor.as<int32_t>();
     d.cu_slots = g->d_cu_slots.as<int32_t>();
-    d.idle_primary = d.cu_slots + CU_KEYS;
+    d.cu_crit = d.cu_slots + CU_KEYS;
+    d.idle_primary = d.cu_slots + 2 * CU_KEYS;
     d.spin_limit = spin_limit;
     return d;
 }

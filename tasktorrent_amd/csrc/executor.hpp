@@ -33,6 +33,7 @@ struct DevGraph {
     int32_t* trace_cursor;
     int32_t* cu_slots;          // [CU_KEYS] workers started per CU (slot 0 = primary)
     int32_t* idle_primary;      // primary workers currently looking for work
+    int32_t* cu_crit;           // [CU_KEYS] critical items running per CU (its other worker waits)
     int64_t spin_limit;         // watchdog in wall_clock64 ticks
 };
 

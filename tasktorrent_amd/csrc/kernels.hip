@@ -70,11 +70,14 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
     // The first worker started on a CU is its primary. A secondary (the second workgroup on that
     // CU) only takes items from a queue that holds more than there are idle primaries, so that
     // scarce work spreads one item per CU instead of pairing up on some CUs while others idle.
-    __shared__ int s_primary;
-    if (tid == 0) s_primary = __hip_atomic_fetch_add(g.cu_slots + cu_key(), 1, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT) == 0;
+    __shared__ int s_primary, s_key;
+    if (tid == 0) {
+        s_key = cu_key();
+        s_primary = __hip_atomic_fetch_add(g.cu_slots + s_key, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+    }
     __syncthreads();
     const bool primary = s_primary != 0;
+    const int my_key = s_key;
     if (primary && tid == 0) __hip_atomic_fetch_add(g.idle_primary, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t rng = blockIdx.x * 2654435761u + 0x9e3779b9u;
     for (;;) {
@@ -86,8 +89,10 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
                 int fb = -1, fh = 0, ft = 0;  // queue to claim from
                 int sb = -1, sh = 0, st = 0;  // first non-empty queue of any XCD (to steal from)
                 bool any_open = false;
-                // a secondary leaves queues that the idle primaries can drain
+                // a secondary leaves queues that the idle primaries can drain; nothing is taken while
+                // the other worker of this CU runs an item of the critical path
                 const int reserve = primary ? 0 : ld_relaxed(g.idle_primary);
+                const bool blocked = ld_relaxed(g.cu_crit + my_key) != 0;
                 for (int base = 0; base < g.n_buckets; base += 64) {  // groups of 64 queues, best first
                     const int b = base + tid;
                     int h = 0, t = 0;
@@ -95,7 +100,7 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
                     if (b < g.n_buckets) {
                         h = ld_relaxed(g.q_head + b);
                         t = ld_relaxed(g.q_tail + b);
-                        ne = t - h > reserve;
+                        ne = !blocked && t - h > reserve;
                         open = h < g.q_base[b + 1] - g.q_base[b];
                         const int home = g.q_home[b];
                         mine = ne && (home < 0 || home == my_xcd);
@@ -181,6 +186,7 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         const int task = (int)(e >> 20), item = (int)(e & 0xFFFFF);
         const DevTask t = g.tasks[task];
+        if (t.crit && tid == 0) __hip_atomic_fetch_add(g.cu_crit + my_key, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         uint64_t t0 = 0;
         if (g.trace && tid == 0) t0 = wall_clock64();
         const bool ran = exec_item<VM>(t, item, g.mats, lds);
@@ -219,6 +225,7 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
                 __hip_atomic_fetch_add(g.done_items, t.n_items, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             if (primary) __hip_atomic_fetch_add(g.idle_primary, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (t.crit) __hip_atomic_fetch_add(g.cu_crit + my_key, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
@@ -235,7 +242,10 @@ __global__ void k_graph_reset_a(DevGraph g, int n_slots) {
         g.q_head[x] = 0;
         g.q_tail[x] = g.tail0[x];
     }
-    for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < CU_KEYS; x += stride) g.cu_slots[x] = 0;
+    for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < CU_KEYS; x += stride) {
+        g.cu_slots[x] = 0;
+        g.cu_crit[x] = 0;
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *g.done_items = 0;
         *g.abort_flag = 0;

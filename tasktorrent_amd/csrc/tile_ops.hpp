@@ -54,6 +54,7 @@ template <> struct Mfma<float> {
 };
 
 template <typename T> __device__ __forceinline__ T zero_val() { return T(0); }
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
 // 16-byte register vectors of a storage type (aligned: LDS side; u: global side, element-aligned).
 template <typename T> struct V16;
@@ -82,14 +83,16 @@ template <> struct V16<float> {
 //      along a column of the output (row = lane % 16): C is then read and written in
 //      16-row segments instead of 16 scattered columns.
 // ---------------------------------------------------------------------------
-template <typename TA, typename TB, typename TC, int BM, int BN, int BK, bool VEC = false, bool SWAP = false>
+// WR: waves stacked along the rows (2: 2x2 waves; 4: each wave owns BM/4 full rows).
+template <typename TA, typename TB, typename TC, int BM, int BN, int BK, bool VEC = false, bool SWAP = false, int WR = 2>
 struct GemmNT {
     static constexpr int NVA = VEC ? V16<TA>::N : 1;  // rows per lane load
     static constexpr int NVB = VEC ? V16<TB>::N : 1;
     static constexpr int PA = BM + (NVA > 2 ? NVA : 2);  // LDS row pitch (16-byte multiple when VEC)
     static constexpr int PB = BN + (NVB > 2 ? NVB : 2);
-    static constexpr int WM = BM / 32;
-    static constexpr int WN = BN / 32;
+    static constexpr int WC = 4 / WR;             // waves along the columns
+    static constexpr int WM = BM / (16 * WR);     // 16x16 blocks per wave
+    static constexpr int WN = BN / (16 * WC);
     static constexpr int LA = BM / NVA;          // lanes per column of the A chunk
     static constexpr int LB = BN / NVB;
     static constexpr int KS_A = NTHREADS / LA;   // columns per load round
@@ -113,11 +116,11 @@ struct GemmNT {
     }
     // Element (row, col) of the output block held by this lane in acc[a][b][r].
     static __device__ __forceinline__ int row_of(int a, int r) {
-        const int lane = threadIdx.x & 63, wr = (threadIdx.x >> 6) >> 1;
+        const int lane = threadIdx.x & 63, wr = (threadIdx.x >> 6) / WC;
         return wr * (WM * 16) + a * 16 + (SWAP ? (lane & 15) : Mfma<TC>::row(lane, r));
     }
     static __device__ __forceinline__ int col_of(int b, int r) {
-        const int lane = threadIdx.x & 63, wc = (threadIdx.x >> 6) & 1;
+        const int lane = threadIdx.x & 63, wc = (threadIdx.x >> 6) % WC;
         return wc * (WN * 16) + b * 16 + (SWAP ? Mfma<TC>::row(lane, r) : (lane & 15));
     }
 
@@ -193,7 +196,7 @@ struct GemmNT {
                                                const TB* __restrict__ B, long ldb, int nv, int K, char* lds) {
         TC* S0 = reinterpret_cast<TC*>(lds);
         const int tid = threadIdx.x;
-        const int lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
+        const int lane = tid & 63, w = tid >> 6, wr = w / WC, wc = w % WC;
         const int nchunks = (K + BK - 1) / BK;
         __syncthreads();
         if (nchunks == 0) return;
@@ -786,7 +789,6 @@ __device__ __forceinline__ void item_trsm(const TS* __restrict__ L, long ldl, co
     }
 }
 
-constexpr int cmax(int a, int b) { return a > b ? a : b; }
 // LDS bytes each item kind needs (the executor reserves the maximum).
 constexpr int LDS_GEMM_F64 = GemmNT<double, double, double, GEMM_BM, GEMM_BM, gemm_bk<double>(), true, true>::LDS_BYTES;
 constexpr int LDS_GEMM_F32 = GemmNT<float, float, float, GEMM_BM, GEMM_BM, gemm_bk<float>(), true, true>::LDS_BYTES;

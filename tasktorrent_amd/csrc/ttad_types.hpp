@@ -36,6 +36,7 @@ struct DevTask {
     int32_t n_items;
     int32_t i, j, k;
     int32_t bucket;
+    int32_t crit;  // on a longest path of the DAG: runs with no other item started on its CU
 };
 
 // Work items per task; host and device must agree.
