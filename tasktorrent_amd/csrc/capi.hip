@@ -39,6 +39,16 @@ struct StreamPlan {
     int vset = N_VSETS - 1;  // executor instantiation covering the plan's item variants
 };
 
+// POTRF tasks (graphs and stream plans) run as one item per 64-row block of the tile (item_potrf_rows,
+// default); TTAD_POTRF_ROWS=0 runs each as one single-workgroup item (for comparisons).
+static bool potrf_rows_enabled() {
+    static const bool on = [] {
+        const char* v = getenv("TTAD_POTRF_ROWS");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+
 static std::shared_ptr<StreamPlan> build_stream_plan(ttad_matrix* M, int prec, int& rc) {
     auto plan = std::make_shared<StreamPlan>();
     std::vector<DevTask> tasks;
@@ -62,7 +72,8 @@ static std::shared_ptr<StreamPlan> build_stream_plan(ttad_matrix* M, int prec, i
         t.i = i;
         t.j = j;
         t.k = k;
-        t.n_items = std::max(1, items_of(kind, M->bsz[i], M->bsz[j]));
+        t.n_items = kind == K_POTRF && potrf_rows_enabled() ? potrf_row_items(M->bsz[j])
+                                                            : std::max(1, items_of(kind, M->bsz[i], M->bsz[j]));
         t.bucket = 0;
         t.crit = 0;
         return t;
@@ -315,12 +326,14 @@ int ttad_matrix_create(ttad_context* ctx, int num_blocks, const int* block_sizes
     TTAD_HIP(M->tab_tiles.alloc((size_t)nt * nt * sizeof(void*)));
     TTAD_HIP(M->tab_dinv.alloc(nt * sizeof(double*)));
     TTAD_HIP(M->info.alloc(sizeof(int)));
+    TTAD_HIP(M->pflags.alloc((size_t)nt * POTRF_FLAGS * sizeof(int32_t)));
     TTAD_HIP(M->desc.alloc(sizeof(DevMat)));
     TTAD_HIP(hipMemcpy(M->tab_bsz.p, M->bsz.data(), nt * 4, hipMemcpyHostToDevice));
     TTAD_HIP(hipMemcpy(M->tab_boff.p, M->boff.data(), (nt + 1) * 4, hipMemcpyHostToDevice));
     TTAD_HIP(hipMemcpy(M->tab_tiles.p, tp.data(), tp.size() * sizeof(void*), hipMemcpyHostToDevice));
     TTAD_HIP(hipMemcpy(M->tab_dinv.p, dp.data(), dp.size() * sizeof(double*), hipMemcpyHostToDevice));
     TTAD_HIP(hipMemset(M->info.p, 0, sizeof(int)));
+    TTAD_HIP(hipMemset(M->pflags.p, 0, (size_t)nt * POTRF_FLAGS * sizeof(int32_t)));
     TTAD_HIP(hipMemset(M->tiles.p, 0, off * M->elem));
     DevMat d{};
     d.nt = nt;
@@ -330,6 +343,7 @@ int ttad_matrix_create(ttad_context* ctx, int num_blocks, const int* block_sizes
     d.tiles = M->tab_tiles.as<void*>();
     d.dinv = M->tab_dinv.as<double*>();
     d.info = M->info.as<int32_t>();
+    d.pflags = M->pflags.as<int32_t>();
     M->host_desc = d;
     TTAD_HIP(hipMemcpy(M->desc.p, &d, sizeof(DevMat), hipMemcpyHostToDevice));
     *out = M.release();
@@ -439,6 +453,7 @@ int ttad_potrf_tiled(ttad_context* ctx, ttad_matrix* M, int schedule, int prio_k
     if (info) *info = 0;
     hipStream_t s = pick(ctx, stream);
     TTAD_HIP(hipMemsetAsync(M->info.p, 0, sizeof(int), s));
+    TTAD_HIP(hipMemsetAsync(M->pflags.p, 0, (size_t)M->nt * POTRF_FLAGS * sizeof(int32_t), s));
     int rc = TTAD_OK;
     ctx->ev_valid = false;
     ctx->kernel_s = -1.0;
@@ -462,7 +477,11 @@ int ttad_potrf_tiled(ttad_context* ctx, ttad_matrix* M, int schedule, int prio_k
         int err = 0;
         TTAD_HIP(hipMemcpyAsync(&err, plan->d_err.p, sizeof(int), hipMemcpyDeviceToHost, s));
         TTAD_HIP(hipStreamSynchronize(s));
-        if (err) return set_error(TTAD_ERR_DEVICE, "potrf_tiled: item variant missing from the executor (%d)", err);
+        if (err) {
+            (void)hipMemset(plan->d_err.p, 0, sizeof(int32_t));
+            if (err == 1) return set_error(TTAD_ERR_TIMEOUT, "potrf_tiled: a row-block POTRF item waited too long");
+            return set_error(TTAD_ERR_DEVICE, "potrf_tiled: item variant missing from the executor (%d)", err);
+        }
     } else if (schedule == TTAD_SCHED_PERSISTENT) {
         rc = cholesky_persistent(ctx, M, prio_kind, prec, s, info);
         if (rc != TTAD_OK && rc != TTAD_ERR_NOT_SPD) return rc;
@@ -662,7 +681,8 @@ static int graph_prepare(ttad_graph* g) {
                                  x.k);
             if ((x.kind == K_GEMM || x.kind == K_SYRK) && ((x.prec == P_F64) != (M->storage == P_F64)))
                 return set_error(TTAD_ERR_INVALID, "graph: task %d precision does not match tile storage", t);
-            x.n_items = std::max(1, items_of(x.kind, M->bsz[x.i], M->bsz[x.j]));
+            x.n_items = x.kind == K_POTRF && potrf_rows_enabled() ? potrf_row_items(M->bsz[x.j])
+                                                                  : std::max(1, items_of(x.kind, M->bsz[x.i], M->bsz[x.j]));
         }
         int needed = 0;
         for (auto& x : g->tasks) needed |= item_variant(x.kind, x.prec, g->mats[x.mat]->storage);
@@ -790,7 +810,11 @@ int ttad_graph_launch(ttad_graph* g, int n_workers, int trace, void* stream) {
     g->stream = pick(ctx, stream);
     launch_graph_reset(d, g->total_items, g->stream);
     for (auto* M : g->mats)
-        if (M) TTAD_HIP(hipMemsetAsync(M->info.p, 0, sizeof(int), g->stream));
+        if (M) {
+            TTAD_HIP(hipMemsetAsync(M->info.p, 0, sizeof(int), g->stream));
+            // row-block POTRF flags: cleared by every completed task, and here after a stopped run
+            TTAD_HIP(hipMemsetAsync(M->pflags.p, 0, (size_t)M->nt * POTRF_FLAGS * sizeof(int32_t), g->stream));
+        }
     TTAD_HIP(hipEventRecord(g->ev_start, g->stream));
     launch_persistent(g->vset, d, g->workers, g->stream);
     TTAD_HIP(hipGetLastError());

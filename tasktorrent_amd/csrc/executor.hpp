@@ -89,8 +89,8 @@ __host__ __device__ inline int item_variant(int kind, int prec, int storage) {
 // LDS of an executor instantiation: the largest item of its variants.
 constexpr int vset_lds(int vm) {
     int b = 0;
-    if (vm & V_F64S) b = cmax(b, cmax(cmax(LDS_GEMM_F64, LDS_POTRF), LDS_TRSM));
-    if (vm & V_F32S) b = cmax(b, cmax(LDS_POTRF, LDS_TRSM_F32));
+    if (vm & V_F64S) b = cmax(b, cmax(cmax(LDS_GEMM_F64, cmax(LDS_POTRF, LDS_POTRF_ROWS)), LDS_TRSM));
+    if (vm & V_F32S) b = cmax(b, cmax(cmax(LDS_POTRF, LDS_POTRF_ROWS), LDS_TRSM_F32));
     if (vm & V_G32) b = cmax(b, LDS_GEMM_F32);
     if (vm & V_GBF16) b = cmax(b, LDS_GEMM_BF16_1);
     if (vm & V_GBF16X3) b = cmax(b, LDS_GEMM_BF16);
@@ -119,6 +119,14 @@ __device__ __attribute__((noinline)) void potrf_call(TS* A, long lda, int n, dou
     item_potrf<TS>(A, lda, n, dinv, info, col_off, lds);
     __builtin_amdgcn_s_setprio(0);
 }
+template <typename TS, int VM>
+__device__ __attribute__((noinline)) void potrf_rows_call(TS* A, long lda, int n, double* dinv, int* info, int col_off,
+                                                          int rb, int32_t* prog, int32_t* err, int64_t spin_limit,
+                                                          char* lds) {
+    __builtin_amdgcn_s_setprio(3);
+    item_potrf_rows<TS>(A, lda, n, dinv, info, col_off, rb, prog, err, spin_limit, lds);
+    __builtin_amdgcn_s_setprio(0);
+}
 template <typename TS, typename TC, int VM>
 __device__ __attribute__((noinline)) void trsm_call(const TS* L, long ldl, const double* dinv, TS* X, long ldx, int m,
                                                     int n, int r0, char* lds) {
@@ -128,9 +136,13 @@ __device__ __attribute__((noinline)) void trsm_call(const TS* L, long ldl, const
 }
 
 // Execute work item `it` of task t. All 256 threads participate. Returns false
-// (doing nothing) for an item variant outside VM.
+// (doing nothing) for an item variant outside VM. A POTRF task of more than one item
+// runs as row-block items that wait on each other (item_potrf_rows): the caller must
+// have every lower-numbered item of the task running or done; their waits give up
+// after spin_limit ticks or once *err is nonzero (and set *err = 1 when they time out).
 template <int VM>
-__device__ __forceinline__ bool exec_item(const DevTask& t, int it, const DevMat* mats, char* lds) {
+__device__ __forceinline__ bool exec_item(const DevTask& t, int it, const DevMat* mats, char* lds, int32_t* err,
+                                          int64_t spin_limit) {
     const DevMat& M = mats[t.mat];
     const int nt = M.nt;
     auto tile = [&](int i, int j) { return M.tiles[i + (long)j * nt]; };
@@ -138,6 +150,22 @@ __device__ __forceinline__ bool exec_item(const DevTask& t, int it, const DevMat
     switch (t.kind) {
         case K_POTRF: {
             const int j = t.j, n = M.bsz[j];
+            if (t.n_items > 1) {
+                int32_t* prog = M.pflags + (long)j * POTRF_FLAGS;
+                if constexpr ((VM & V_F64S) != 0)
+                    if (f64s) {
+                        potrf_rows_call<double, VM>((double*)tile(j, j), n, n, M.dinv[j], M.info, M.boff[j], it, prog,
+                                                    err, spin_limit, lds);
+                        return true;
+                    }
+                if constexpr ((VM & V_F32S) != 0)
+                    if (!f64s) {
+                        potrf_rows_call<float, VM>((float*)tile(j, j), n, n, M.dinv[j], M.info, M.boff[j], it, prog,
+                                                   err, spin_limit, lds);
+                        return true;
+                    }
+                return false;
+            }
             if constexpr ((VM & V_F64S) != 0)
                 if (f64s) {
                     potrf_call<double, VM>((double*)tile(j, j), n, n, M.dinv[j], M.info, M.boff[j], lds);

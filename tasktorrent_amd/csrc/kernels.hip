@@ -22,7 +22,9 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_items(const De
         else hi = mid - 1;
     }
     const DevTask t = tasks[lo];
-    if (!exec_item<VM>(t, g - prefix[lo], mats, lds) && threadIdx.x == 0) atomicCAS(err, 0, 3);
+    // row-block POTRF items wait only on lower-numbered blocks of the same launch, which
+    // are dispatched first
+    if (!exec_item<VM>(t, g - prefix[lo], mats, lds, err, (int64_t)500000000) && threadIdx.x == 0) atomicCAS(err, 0, 3);
 }
 
 // ---------------------------------------------------------------------------
@@ -189,7 +191,7 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
         if (t.crit && tid == 0) __hip_atomic_fetch_add(g.cu_crit + my_key, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         uint64_t t0 = 0;
         if (g.trace && tid == 0) t0 = wall_clock64();
-        const bool ran = exec_item<VM>(t, item, g.mats, lds);
+        const bool ran = exec_item<VM>(t, item, g.mats, lds, g.abort_flag, g.spin_limit);
         __syncthreads();
         if (tid == 0) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");

@@ -312,7 +312,7 @@ __device__ __forceinline__ void srv_exec(const DevServer& s, const SrvTask& t, i
             d.i = t.i;
             d.j = t.j;
             d.k = t.k;
-            (void)exec_item<V_ALL>(d, it, s.mats, lds);  // request graphs only hold supported kinds
+            (void)exec_item<V_ALL>(d, it, s.mats, lds, s.err, s.spin_limit);  // request graphs only hold supported kinds
             break;
         }
     }

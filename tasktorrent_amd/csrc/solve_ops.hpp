@@ -181,30 +181,6 @@ __device__ __forceinline__ void block_mma(typename Mfma<TC>::acc_t (&acc)[NR / 1
     }
 }
 
-// Waits until *count >= want and returns the value seen. The polls are relaxed (an acquire
-// load invalidates the L2 on every iteration, which with hundreds of waiting workgroups
-// evicts everyone's data); one agent-scope acquire follows. Gives up after `limit` ticks
-// (sets *err = 1) or as soon as *err is nonzero (an earlier wait timed out, or the owner
-// asked the kernel to stop), so that a failed run drains in one time limit. `slow` polls
-// about once a microsecond instead of back to back.
-__device__ __forceinline__ int wait_count(const int32_t* count, int want, int32_t* err, int64_t limit,
-                                          bool slow = false) {
-    const int64_t t0 = (int64_t)wall_clock64();
-    int v;
-    while ((v = __hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < want) {
-        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0) { v = want; break; }
-        if ((int64_t)wall_clock64() - t0 > limit) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            v = want;
-            break;
-        }
-        if (slow) __builtin_amdgcn_s_sleep(32);
-        else __builtin_amdgcn_s_sleep(1);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return v;
-}
-
 // One 64-row block p of a triangular sweep with the factor held in M.
 // forward != 0: Y[p] = L[p,p]^{-1} (R[p] - sum_{q<p} L[p,q] Y[q])          (R FP64, Y in TC)
 // forward == 0: D[p] = L[p,p]^{-T} (Y[p] - sum_{q>p} L[q,p]^T D[q]), X[p] += D[p] (FP64)
@@ -247,10 +223,17 @@ __device__ __forceinline__ void item_sweep(const DevMat& M, const SweepTab& tab,
         }
     {
         const TTAD_GLOBAL double* Dv = gmem(M.dinv[tab.blk_tile[p]]) + (long)(tab.blk_off[p] / SB) * SB * SB;
-        for (int e = threadIdx.x; e < SB * SB; e += NTHREADS) {
-            const int a = e % SB, b = e / SB;  // Dv[a + b*64] = Dinv(a, b)
-            if (forward) Ds[b * LD + a] = (TC)Dv[e];   // (m, k) = Dinv(m, k)
-            else Ds[a * LD + b] = (TC)Dv[e];           // (m, k) = Dinv(k, m)
+        // every load in flight before the first LDS store
+        constexpr int PER = SB * SB / NTHREADS;
+        const int a = threadIdx.x % SB, b0 = threadIdx.x / SB;  // Dv[a + b*64] = Dinv(a, b)
+        double dv[PER];
+#pragma unroll
+        for (int q = 0; q < PER; q++) dv[q] = Dv[a + (b0 + q * (NTHREADS / SB)) * SB];
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int b = b0 + q * (NTHREADS / SB);
+            if (forward) Ds[b * LD + a] = (TC)dv[q];   // (m, k) = Dinv(m, k)
+            else Ds[a * LD + b] = (TC)dv[q];           // (m, k) = Dinv(k, m)
         }
     }
 

@@ -688,22 +688,39 @@ __device__ __forceinline__ void item_potrf(TS* __restrict__ A, long lda, int n, 
             }
             __syncthreads();
         }
-        // diagonal block into LDS (identity padding), factor + inverse
-        for (int e = tid; e < DB * DB; e += NTHREADS) {
-            const int i = e % DB, j = e / DB;
-            double v;
-            if (i < pw && j < pw) v = (i >= j) ? (double)gmem(A)[(p0 + i) + (long)(p0 + j) * lda] : 0.0;
-            else v = (i == j) ? 1.0 : 0.0;
-            D[i + j * DP] = v;
+        // diagonal block into LDS (identity padding), every load in flight before the
+        // first LDS store; then factor + inverse
+        {
+            constexpr int PER = DB * DB / NTHREADS;
+            const int i = tid % DB, j0 = tid / DB;
+            TTAD_GLOBAL const TS* Ac = gmem(A) + (p0 + i) + (long)p0 * lda;
+            TS v[PER];
+#pragma unroll
+            for (int q = 0; q < PER; q++) {
+                const int j = j0 + q * (NTHREADS / DB);
+                v[q] = (i < pw && j < pw && i >= j) ? Ac[(long)j * lda] : TS(0);
+            }
+#pragma unroll
+            for (int q = 0; q < PER; q++) {
+                const int j = j0 + q * (NTHREADS / DB);
+                D[i + j * DP] = (i < pw && j < pw) ? (double)v[q] : (i == j) ? 1.0 : 0.0;
+            }
         }
         __syncthreads();
         const int bad = lds_potrf_inv64(D, X, pw);
         if (bad && tid == 0) atomicCAS(info, 0, col_off + p0 + bad);
         double* Xinv = dinv + (long)pb * DB * DB;
-        for (int e = tid; e < DB * DB; e += NTHREADS) {
-            const int i = e % DB, j = e / DB;
-            if (i < pw && j < pw && i >= j) gmem(A)[(p0 + i) + (long)(p0 + j) * lda] = (TS)D[i + j * DP];
-            gmem(Xinv)[e] = X[i + j * DP];
+        {
+            constexpr int PER = DB * DB / NTHREADS;
+            const int i = tid % DB, j0 = tid / DB;
+            TTAD_GLOBAL TS* Ac = gmem(A) + (p0 + i) + (long)p0 * lda;
+#pragma unroll
+            for (int q = 0; q < PER; q++) {
+                const int j = j0 + q * (NTHREADS / DB);
+                const double dv = D[i + j * DP], xv = X[i + j * DP];
+                if (i < pw && j < pw && i >= j) Ac[(long)j * lda] = (TS)dv;
+                gmem(Xinv)[i + j * DB] = xv;
+            }
         }
         __syncthreads();
         // rows below: L(q, p) = A(q, p) Xinv^T
@@ -724,6 +741,184 @@ __device__ __forceinline__ void item_potrf(TS* __restrict__ A, long lda, int n, 
                     }
         }
         __syncthreads();
+    }
+}
+
+// Waits until *count >= want and returns the value seen. The polls are relaxed (an acquire
+// load invalidates the L2 on every iteration, which with hundreds of waiting workgroups
+// evicts everyone's data); one agent-scope acquire follows. Gives up after `limit` ticks
+// (sets *err = 1) or as soon as *err is nonzero (an earlier wait timed out, or the owner
+// asked the kernel to stop), so that a failed run drains in one time limit. `slow` polls
+// about once a microsecond instead of back to back.
+__device__ __forceinline__ int wait_count(const int32_t* count, int want, int32_t* err, int64_t limit,
+                                          bool slow = false) {
+    const int64_t t0 = (int64_t)wall_clock64();
+    int v;
+    while ((v = __hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < want) {
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0) { v = want; break; }
+        if ((int64_t)wall_clock64() - t0 > limit) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            v = want;
+            break;
+        }
+        if (slow) __builtin_amdgcn_s_sleep(32);
+        else __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return v;
+}
+
+
+// ---------------------------------------------------------------------------
+// POTRF split into row-block items (used by the persistent executor): item rb of the
+// task owns rows [64 rb, 64 rb + 64) of the n x n tile and computes L(rb, 0..rb) by a
+// right-looking pass over the 64-column panels j < rb:
+//   L(rb,j)   = A(rb,j) Xinv(j)^T          once panel j is published
+//   A(rb,rb) -= L(rb,j) L(rb,j)^T          kept in registers
+//   A(rb,q)  -= L(rb,j) L(q,j)^T, j<q<rb  once L(q,j) is published
+// and then factors and inverts its diagonal block (lds_potrf_inv64) and publishes
+// Xinv(rb). prog[rb] counts the panels whose part of row block rb is final (rb + 1:
+// the whole row block, diagonal block and its inverse included). An item waits only on
+// lower-numbered items of its task, which are claimed before it, so every wait ends.
+// From one diagonal block to the next the chain is one 64^3 product, one 64^3 update
+// and the 64x64 factorization, where the single-workgroup item_potrf runs every panel
+// GEMM of the tile in sequence. Same arithmetic (FP64, operands rounded to the tile's
+// storage type as stored) and outputs (L in the tile, dinv) as item_potrf.
+// ---------------------------------------------------------------------------
+constexpr int POTRF_FLAGS = 128;  // ints of progress flags per diagonal tile
+constexpr int POTRF_FIN = POTRF_FLAGS - 1;  // completion count (clears the flags for the next run)
+__host__ __device__ inline int potrf_row_items(int n) { return (n + DB - 1) / DB; }
+
+// Dst[k * P + m] = (double) T[m + k * ld] for m < rows, k < cols (else 0); 64 x 64, every
+// global load in flight before the first LDS store.
+template <typename TS, int P>
+__device__ __forceinline__ void stage64(double* Dst, const TTAD_GLOBAL TS* T, long ld, int rows, int cols) {
+    constexpr int PER = DB * DB / NTHREADS;
+    const int m = threadIdx.x % DB, k0 = threadIdx.x / DB;
+    TS v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int k = k0 + q * (NTHREADS / DB);
+        v[q] = (m < rows && k < cols) ? T[m + (long)k * ld] : TS(0);
+    }
+#pragma unroll
+    for (int q = 0; q < PER; q++) Dst[(k0 + q * (NTHREADS / DB)) * P + m] = (double)v[q];
+}
+
+template <typename TS>
+__device__ __forceinline__ void item_potrf_rows(TS* __restrict__ A, long lda, int n, double* __restrict__ dinv,
+                                                int* info, int col_off, int rb, int32_t* prog, int32_t* err,
+                                                int64_t spin_limit, char* lds) {
+    using G3 = GemmNT<double, double, double, DB, DB, DB, false, true>;  // 64x64x64, both operands in LDS
+    using acc_t = typename G3::acc_t;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w / G3::WC, wc = w % G3::WC;
+    const int nrb = potrf_row_items(n);
+    const int p0 = rb * DB, pw = min(DB, n - p0);
+    double* At = reinterpret_cast<double*>(lds);  // At[k * PA + m] = first operand (m, k)
+    double* Bt = At + DB * G3::PA;                // Bt[k * PB + c] = second operand (c, k)
+    TTAD_GLOBAL TS* Ag = gmem(A);
+    TTAD_GLOBAL TS* Arow = Ag + p0;  // row block rb
+    acc_t accd[G3::WM][G3::WN], acc[G3::WM][G3::WN];
+    // accd = -A(rb, rb), lower part
+#pragma unroll
+    for (int a = 0; a < G3::WM; a++)
+#pragma unroll
+        for (int b = 0; b < G3::WN; b++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int rr = G3::row_of(a, r), cc = G3::col_of(b, r);
+                accd[a][b][r] = (rr < pw && cc < pw && rr >= cc) ? -(double)Arow[rr + (long)(p0 + cc) * lda] : 0.0;
+            }
+    for (int j = 0; j < rb; j++) {
+        const int c0 = j * DB;  // panel j < rb is a full 64 columns
+        if (tid == 0) wait_count(prog + j, j + 1, err, spin_limit);
+        __syncthreads();  // Xinv(j) and L(j, 0..j) final; At / Bt free
+        stage64<TS, G3::PA>(At, Arow + (long)c0 * lda, lda, pw, DB);
+        stage64<double, G3::PB>(Bt, gmem(dinv) + (long)j * DB * DB, DB, DB, DB);
+        __syncthreads();
+        G3::zero(acc);
+        G3::compute(acc, At, lane, wr, wc);
+        __syncthreads();
+        // L(rb, j) into the tile and, as stored, into both operand buffers
+#pragma unroll
+        for (int a = 0; a < G3::WM; a++)
+#pragma unroll
+            for (int b = 0; b < G3::WN; b++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int rr = G3::row_of(a, r), cc = G3::col_of(b, r);
+                    const TS v = (TS)acc[a][b][r];
+                    if (rr < pw) Arow[rr + (long)(c0 + cc) * lda] = v;
+                    At[cc * G3::PA + rr] = (double)v;
+                    Bt[cc * G3::PB + rr] = (double)v;
+                }
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(prog + rb, j + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        G3::compute(accd, At, lane, wr, wc);  // -A(rb,rb) + L(rb,j) L(rb,j)^T
+        // blocks of this row to the right of panel j: A(rb,q) -= L(rb,j) L(q,j)^T
+        for (int q = j + 1; q < rb; q++) {
+            if (tid == 0) wait_count(prog + q, j + 1, err, spin_limit);
+            __syncthreads();  // L(q, j) final; Bt free
+            stage64<TS, G3::PB>(Bt, Ag + q * DB + (long)c0 * lda, lda, DB, DB);
+            TTAD_GLOBAL TS* C = Arow + (long)q * DB * lda;
+#pragma unroll
+            for (int a = 0; a < G3::WM; a++)
+#pragma unroll
+                for (int b = 0; b < G3::WN; b++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int rr = G3::row_of(a, r), cc = G3::col_of(b, r);
+                        acc[a][b][r] = rr < pw ? -(double)C[rr + (long)cc * lda] : 0.0;
+                    }
+            __syncthreads();
+            G3::compute(acc, At, lane, wr, wc);
+#pragma unroll
+            for (int a = 0; a < G3::WM; a++)
+#pragma unroll
+                for (int b = 0; b < G3::WN; b++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int rr = G3::row_of(a, r), cc = G3::col_of(b, r);
+                        if (rr < pw) C[rr + (long)cc * lda] = (TS)(-acc[a][b][r]);
+                    }
+        }
+    }
+    // diagonal block (identity padding): factor and invert in LDS
+    double* D = At;
+    double* X = D + DB * DP;
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < G3::WM; a++)
+#pragma unroll
+        for (int b = 0; b < G3::WN; b++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int rr = G3::row_of(a, r), cc = G3::col_of(b, r);
+                D[rr + cc * DP] = (rr < pw && cc < pw) ? (rr >= cc ? -accd[a][b][r] : 0.0) : (rr == cc ? 1.0 : 0.0);
+            }
+    __syncthreads();
+    const int bad = lds_potrf_inv64(D, X, pw);
+    if (bad && tid == 0) atomicCAS(info, 0, col_off + p0 + bad);
+    {
+        constexpr int PER = DB * DB / NTHREADS;
+        const int i = tid % DB, j0 = tid / DB;
+        TTAD_GLOBAL double* Xg = gmem(dinv) + (long)rb * DB * DB;
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int j = j0 + q * (NTHREADS / DB);
+            const double dv = D[i + j * DP], xv = X[i + j * DP];
+            if (i < pw && j < pw && i >= j) Arow[i + (long)(p0 + j) * lda] = (TS)dv;
+            Xg[i + j * DB] = xv;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        __hip_atomic_store(prog + rb, rb + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        // the last item of the task to finish clears the flags for the next run on this tile
+        if (__hip_atomic_fetch_add(prog + POTRF_FIN, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == nrb - 1) {
+            for (int i = 0; i < nrb; i++) __hip_atomic_store(prog + i, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(prog + POTRF_FIN, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 
@@ -771,8 +966,23 @@ __device__ __forceinline__ void item_trsm(const TS* __restrict__ L, long ldl, co
             for (int b = 0; b < G::WN; b++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) At[G::col_of(b, r) * G3::PA + G::row_of(a, r)] = -acc[a][b][r];
-        const double* Dv = dinv + (long)cb * DB * DB;  // Dv[i + j*DB] = Dinv(i, j)
-        for (int e = threadIdx.x; e < DB * DB; e += NTHREADS) Bt[(e / DB) * G3::PB + (e % DB)] = (TC)gmem(Dv)[e];
+        // Dinv block into Bt, every global load in flight before the first LDS store (a
+        // load-store loop would wait for each load in turn)
+        {
+            constexpr int PER = DB * DB / (2 * NTHREADS);
+            using d2 = V16<double>::t;
+            const TTAD_GLOBAL d2* Dv = reinterpret_cast<const TTAD_GLOBAL d2*>(gmem(dinv) + (long)cb * DB * DB);
+            d2 v[PER];  // Dv[i + j*DB] = Dinv(i, j), two rows of a column per element
+#pragma unroll
+            for (int q = 0; q < PER; q++) v[q] = Dv[threadIdx.x + q * NTHREADS];
+#pragma unroll
+            for (int q = 0; q < PER; q++) {
+                const int e = 2 * (threadIdx.x + q * NTHREADS);
+                TC* d = Bt + (e / DB) * G3::PB + (e % DB);
+                d[0] = (TC)v[q].x;
+                d[1] = (TC)v[q].y;
+            }
+        }
         __syncthreads();
         G3::zero(acc);
         G3::compute(acc, At, lane, wr, wc);
@@ -800,6 +1010,7 @@ template <typename TS, int PREC> constexpr int gemm_item_lds() {
 }
 constexpr int LDS_POTRF = PotrfGemm<double>::G::LDS_BYTES > 2 * DB * DP * 8 ? PotrfGemm<double>::G::LDS_BYTES
                                                                              : 2 * DB * DP * 8;
+constexpr int LDS_POTRF_ROWS = GemmNT<double, double, double, DB, DB, DB, false, true>::STAGE * 8;
 constexpr int LDS_TRSM = cmax(GemmNT<double, double, double, TRSM_BM, DB, 32, true, true>::LDS_BYTES,
                               GemmNT<double, double, double, TRSM_BM, DB, DB, false, true>::STAGE * 8);
 constexpr int LDS_TRSM_F32 = cmax(GemmNT<float, float, float, TRSM_BM, DB, 32, true, true>::LDS_BYTES,

@@ -25,7 +25,7 @@ struct DevMat {
     void* const* tiles;    // [nt*nt], entry i + j*nt
     double* const* dinv;   // [nt]: inverses of the 64x64 diagonal blocks of L_jj (FP64, 64x64 each, ld 64)
     int32_t* info;         // first non-positive pivot, 1-based global column; 0 = none
-    int32_t* accbuf_ok;    // unused slot kept for layout stability
+    int32_t* pflags;       // [nt * 128]: progress flags of the row-block POTRF items of each diagonal tile (zero between runs)
 };
 
 // 32-byte device task record.
