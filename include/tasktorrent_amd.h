@@ -118,7 +118,8 @@ int ttad_matrix_tile(const ttad_matrix* M, int i, int j, void** ptr, int* ld);
 /* Replaces cholesky() of 2d_cholesky.cpp:41-690 on one rank, non-accumulate mode:
  * the whole factorization of M in place (lower tiles become L).
  * prec: TTAD_F64 (M FP64) or TTAD_F32 / TTAD_BF16 / TTAD_BF16X3 for the trailing update (M FP32;
- * diagonal POTRF in FP64). prio_kind 0..3 as PrioKind (:39, :157-204).
+ * diagonal POTRF in FP64). prio_kind 0..3 as PrioKind (:39, :157-204); 4 (no counterpart in
+ * the reference): longest remaining path of each task in the DAG ("bottom level").
  * Synchronous; *info as ttad_dpotrf. */
 int ttad_potrf_tiled(ttad_context* ctx, ttad_matrix* M, int schedule, int prio_kind, int prec, void* stream,
                      int* info);

@@ -447,7 +447,7 @@ int ttad_matrix_tile(const ttad_matrix* M, int i, int j, void** ptr, int* ld) {
 // ---------------------------------------------------------------------------
 int ttad_potrf_tiled(ttad_context* ctx, ttad_matrix* M, int schedule, int prio_kind, int prec, void* stream,
                      int* info) {
-    if (!ctx || !M || prio_kind < 0 || prio_kind > 3) return set_error(TTAD_ERR_INVALID, "potrf_tiled: bad args");
+    if (!ctx || !M || prio_kind < 0 || prio_kind > 4) return set_error(TTAD_ERR_INVALID, "potrf_tiled: bad args");
     if ((prec == TTAD_F64) != (M->storage == TTAD_F64) || prec < 0 || prec > 3)
         return set_error(TTAD_ERR_INVALID, "potrf_tiled: precision %d does not match the tile storage", prec);
     if (info) *info = 0;
@@ -708,6 +708,7 @@ static int graph_prepare(ttad_graph* g) {
         TTAD_HIP(g->d_tail0.alloc(g->n_buckets * 4));
         TTAD_HIP(g->d_q_head.alloc(g->n_buckets * 4));
         TTAD_HIP(g->d_q_tail.alloc(g->n_buckets * 4));
+        TTAD_HIP(g->d_q_count.alloc(g->n_buckets * 4));
         TTAD_HIP(g->d_q_base.alloc((g->n_buckets + 1) * 4));
         TTAD_HIP(g->d_q_home.alloc(g->n_buckets * 4));
         TTAD_HIP(g->d_q_items.alloc((size_t)g->total_items * 8));
@@ -773,6 +774,7 @@ static DevGraph graph_image(ttad_graph* g, bool trace, int64_t spin_limit) {
     d.items_left = g->d_items_left.as<int32_t>();
     d.q_head = g->d_q_head.as<int32_t>();
     d.q_tail = g->d_q_tail.as<int32_t>();
+    d.q_count = g->d_q_count.as<int32_t>();
     d.q_base = g->d_q_base.as<int32_t>();
     d.q_home = g->d_q_home.as<int32_t>();
     d.q_items = g->d_q_items.as<int64_t>();

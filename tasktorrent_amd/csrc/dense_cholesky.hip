@@ -1,9 +1,11 @@
 // The tiled dense Cholesky PTG of miniapp/dense_cholesky/2d_cholesky.cpp:41-690
 // (one rank, non-accumulate mode), written against the Taskflow<K>/Threadpool
 // drop-in API and compiled into a persistent device graph.
+#include <algorithm>
 #include <array>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "../../include/tasktorrent_amd/taskflow.hpp"
 #include "runtime.hpp"
@@ -33,10 +35,43 @@ static double prio_gemm(int kind, int nb, int k, int i, int j) {
     return 1.0;
 }
 
+// Priority kind 4 (this library's, not the reference's): the length of the longest path from a
+// task to the end of the DAG ("bottom level"), with task weights POTRF 2, TRSM 2, GEMM/SYRK 1
+// (their measured latencies on the critical path, in units of one GEMM item). Unlike kind 2,
+// which ranks every update of column j alike, it also counts the serial chain of updates
+// GEMM(i,j,0..j-1) of each tile, so the early updates of far tiles are not left to the end.
+// Closed form of the bottom levels of the Cholesky PTG (dependencies of 2d_cholesky.cpp):
+//   GEMM(i,j,k): (j-k) wG + (i == j ? P(j) : T(i,j))     (the rest of the tile's chain)
+//   T(i,k) = wT + max((i-k) wG + P(i), wG + T(i,k+1))     (via column i, or via row i)
+//   P(k)   = wP + max_{i>k} T(i,k),  P(nb-1) = wP
+struct BottomLevels {
+    static constexpr double wP = 2, wT = 2, wG = 1;
+    int nb;
+    std::vector<double> P, T;  // T[i + k*nb], i > k
+    explicit BottomLevels(int n) : nb(n), P(n, 0.0), T((size_t)n * n, 0.0) {
+        for (int k = nb - 1; k >= 0; k--) {
+            // T(i,k) for i > k needs P(i) (i > k, done) and T(i,k+1) (done: column k+1 first)
+            double best = 0.0;
+            for (int i = nb - 1; i > k; i--) {
+                double v = (i - k) * wG + P[i];
+                if (k + 1 < i) v = std::max(v, wG + T[i + (size_t)(k + 1) * nb]);
+                T[i + (size_t)k * nb] = wT + v;
+                best = std::max(best, T[i + (size_t)k * nb]);
+            }
+            P[k] = wP + best;
+        }
+    }
+    double potrf(int j) const { return P[j]; }
+    double trsm(int i, int j) const { return T[i + (size_t)j * nb]; }
+    double gemm(int k, int i, int j) const { return (j - k) * wG + (i == j ? P[j] : T[i + (size_t)j * nb]); }
+};
+
 // Builds the DAG through the Taskflow API exactly as the reference miniapp declares it.
 // ctx == nullptr: dry compile into `dry` (no device).
 static int build_cholesky_graph(ttad_context* ctx, int nb, int storage, int prec, int prio_kind, ttad_graph** out,
                                 Threadpool** dry = nullptr) {
+    std::unique_ptr<BottomLevels> bl;
+    if (prio_kind == 4) bl = std::make_unique<BottomLevels>(nb);
     auto tp_owned = std::make_unique<Threadpool>(ctx);
     Threadpool& tp = *tp_owned;
     Taskflow<int> potrf(&tp);
@@ -49,7 +84,7 @@ static int build_cholesky_graph(ttad_context* ctx, int nb, int storage, int prec
             for (int i = j + 1; i < nb; i++) trsm.fulfill_promise({i, j});
         })
         .set_indegree([](int) { return 1; })
-        .set_priority([&](int j) { return prio_potrf(prio_kind, nb, j); })
+        .set_priority([&](int j) { return bl ? bl->potrf(j) : prio_potrf(prio_kind, nb, j); })
         .set_mapping([&](int j) { return block_2_thread(j, j); })
         .set_name([](int j) { return "POTRF_" + std::to_string(j); });
 
@@ -60,7 +95,7 @@ static int build_cholesky_graph(ttad_context* ctx, int nb, int storage, int prec
             for (int k = i; k < nb; k++) gemm.fulfill_promise({j, k, i});
         })
         .set_indegree([](int2 ij) { return 1 + (ij[1] == 0 ? 0 : 1); })
-        .set_priority([&](int2 ij) { return prio_trsm(prio_kind, nb, ij[0], ij[1]); })
+        .set_priority([&](int2 ij) { return bl ? bl->trsm(ij[0], ij[1]) : prio_trsm(prio_kind, nb, ij[0], ij[1]); })
         .set_mapping([&](int2 ij) { return block_2_thread(ij[0], ij[1]); })
         .set_name([](int2 ij) { return "TRSM_" + std::to_string(ij[0]) + "_" + std::to_string(ij[1]); });
 
@@ -74,7 +109,9 @@ static int build_cholesky_graph(ttad_context* ctx, int nb, int storage, int prec
             else trsm.fulfill_promise({i, j});
         })
         .set_indegree([](int3 kij) { return (kij[1] == kij[2] ? 1 : 2) + (kij[0] == 0 ? 0 : 1); })
-        .set_priority([&](int3 kij) { return prio_gemm(prio_kind, nb, kij[0], kij[1], kij[2]); })
+        .set_priority([&](int3 kij) {
+            return bl ? bl->gemm(kij[0], kij[1], kij[2]) : prio_gemm(prio_kind, nb, kij[0], kij[1], kij[2]);
+        })
         .set_mapping([&](int3 kij) { return block_2_thread(kij[1], kij[2]); })
         .set_binding([](int3) { return false; })
         .set_name([](int3 kij) {
@@ -130,7 +167,7 @@ int cholesky_dag_compile(int nb, int prio_kind, std::vector<ttad_task_desc>& des
 extern "C" int ttad_cholesky_dag(int num_blocks, int prio_kind, int* n_tasks, int* n_edges, ttad_task_desc* tasks,
                                  int32_t* indegree, double* priority, int32_t* out_ptr, int32_t* out_idx) {
     using namespace ttad;
-    if (num_blocks <= 0 || num_blocks > 4096 || prio_kind < 0 || prio_kind > 3 || !n_tasks || !n_edges)
+    if (num_blocks <= 0 || num_blocks > 4096 || prio_kind < 0 || prio_kind > 4 || !n_tasks || !n_edges)
         return set_error(TTAD_ERR_INVALID, "cholesky_dag: bad args");
     Threadpool* tp = nullptr;
     int rc = build_cholesky_graph(nullptr, num_blocks, TTAD_F64, TTAD_F64, prio_kind, nullptr, &tp);

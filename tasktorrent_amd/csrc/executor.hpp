@@ -23,6 +23,8 @@ struct DevGraph {
     int32_t* items_left;
     int32_t* q_head;            // [n_buckets]
     int32_t* q_tail;            // [n_buckets]
+    int32_t* q_count;           // [n_buckets] entries pushed and not yet claimed (dips below zero while
+                                // a failed claim gives its credit back)
     const int32_t* q_base;      // [n_buckets+1]
     const int32_t* q_home;      // [n_buckets] -1 = any worker, else the XCD served first
     int64_t* q_items;           // entries (task << 20) | item; -1 = not yet written

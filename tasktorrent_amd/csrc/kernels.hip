@@ -35,9 +35,12 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_items(const De
 //   wave 0 reads the heads/tails of the queues 64 at a time (one per lane) and claims
 //   the head entry of the first non-empty queue that is shared or homed on its own XCD;
 //   when there is none it takes the first non-empty queue of another XCD (stealing).
-//   The claim is a fetch-add when the queue holds more entries than twice the number
-//   of workers (no claim can then pass the tail), else a CAS (losers back off a random
-//   interval); the workgroup runs that work item, then thread 0 decrements the
+//   A claim takes one credit from the queue's count of unclaimed entries (fetch-add -1;
+//   a claimer that finds none gives it back) and then its slot (fetch-add on the head):
+//   the credits never exceed the pushed entries, so a slot never passes the tail, and
+//   no claim retries a compare-and-swap (CAS claims near the tail, with hundreds of
+//   workers on few queues, cost more than half the executor's time in measurements).
+//   The workgroup runs that work item, then thread 0 decrements the
 //   task's remaining-items counter; the last item of a task decrements its
 //   successors' dependency counters and pushes every item of a successor that
 //   reaches zero into its queue. A worker exits when every queue is fully claimed.
@@ -60,6 +63,8 @@ __device__ __forceinline__ void push_task(const DevGraph& g, int s) {
     for (int r = 0; r < t.n_items; r++)
         __hip_atomic_store(g.q_items + g.q_base[b] + pos + r, ((int64_t)s << 20) | r, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
+    // claim credits (a claimer that gets a slot whose entry is not written yet polls it)
+    __hip_atomic_fetch_add(g.q_count + b, t.n_items, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <int VM>
@@ -67,7 +72,6 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
     extern __shared__ __align__(16) char lds[];
     __shared__ int64_t s_item;
     const int tid = threadIdx.x;
-    const int margin = 2 * (int)gridDim.x;  // more queued items than this: a fetch-add claim cannot overshoot
     const int my_xcd = xcc_id();
     // The first worker started on a CU is its primary. A secondary (the second workgroup on that
     // CU) only takes items from a queue that holds more than there are idle primaries, so that
@@ -88,8 +92,8 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
             int64_t got = -1;
             const int64_t t_start = (int64_t)wall_clock64();
             for (;;) {
-                int fb = -1, fh = 0, ft = 0;  // queue to claim from
-                int sb = -1, sh = 0, st = 0;  // first non-empty queue of any XCD (to steal from)
+                int fb = -1;  // queue to claim from
+                int sb = -1;  // first non-empty queue of any XCD (to steal from)
                 bool any_open = false;
                 // a secondary leaves queues that the idle primaries can drain; nothing is taken while
                 // the other worker of this CU runs an item of the critical path
@@ -97,50 +101,33 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
                 const bool blocked = ld_relaxed(g.cu_crit + my_key) != 0;
                 for (int base = 0; base < g.n_buckets; base += 64) {  // groups of 64 queues, best first
                     const int b = base + tid;
-                    int h = 0, t = 0;
                     bool ne = false, open = false, mine = false;
                     if (b < g.n_buckets) {
-                        h = ld_relaxed(g.q_head + b);
-                        t = ld_relaxed(g.q_tail + b);
-                        ne = !blocked && t - h > reserve;
+                        const int h = ld_relaxed(g.q_head + b);
+                        ne = !blocked && ld_relaxed(g.q_count + b) > reserve;
                         open = h < g.q_base[b + 1] - g.q_base[b];
                         const int home = g.q_home[b];
                         mine = ne && (home < 0 || home == my_xcd);
                     }
                     const unsigned long long mask = __ballot(mine), any = __ballot(ne);
                     any_open |= __ballot(open) != 0;
-                    if (sb < 0 && any) {
-                        const int l = __ffsll((long long)any) - 1;
-                        sb = base + l;
-                        sh = __shfl(h, l);
-                        st = __shfl(t, l);
-                    }
+                    if (sb < 0 && any) sb = base + __ffsll((long long)any) - 1;
                     if (mask) {
-                        const int l = __ffsll((long long)mask) - 1;
-                        fb = base + l;
-                        fh = __shfl(h, l);
-                        ft = __shfl(t, l);
+                        fb = base + __ffsll((long long)mask) - 1;
                         break;
                     }
                 }
-                if (fb < 0 && sb >= 0) {  // nothing for this XCD: steal
-                    fb = sb;
-                    fh = sh;
-                    ft = st;
-                }
+                if (fb < 0 && sb >= 0) fb = sb;  // nothing for this XCD: steal
                 const bool all_claimed = fb < 0 && !any_open;
                 if (fb >= 0) {
                     int64_t e = -1;
                     if (tid == 0) {
                         int slot = -1;
-                        if (ft - fh > margin) {
+                        if (__hip_atomic_fetch_add(g.q_count + fb, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >
+                            reserve)
                             slot = __hip_atomic_fetch_add(g.q_head + fb, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        } else {
-                            int hh = fh;
-                            if (__hip_atomic_compare_exchange_strong(g.q_head + fb, &hh, fh + 1, __ATOMIC_RELAXED,
-                                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                                slot = fh;
-                        }
+                        else  // no credit left for this worker: give it back
+                            __hip_atomic_fetch_add(g.q_count + fb, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         if (slot >= 0) {
                             const int64_t* sp = g.q_items + g.q_base[fb] + slot;
                             // claimed before its producer wrote it: poll relaxed (an acquire load would
@@ -160,7 +147,8 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
                         got = e;
                         break;
                     }
-                    // lost the CAS: back off a random 0-15 x 64 cycles so the losers do not retry in lockstep
+                    // lost the race for the last credits: back off a random 0-15 x 64 cycles so the losers do
+                    // not retry in lockstep
                     rng = rng * 1664525u + 1013904223u;
                     for (uint32_t q = rng >> 28; q > 0; q--) __builtin_amdgcn_s_sleep(1);
                     continue;
@@ -243,6 +231,7 @@ __global__ void k_graph_reset_a(DevGraph g, int n_slots) {
     for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < g.n_buckets; x += stride) {
         g.q_head[x] = 0;
         g.q_tail[x] = g.tail0[x];
+        g.q_count[x] = g.tail0[x];
     }
     for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < CU_KEYS; x += stride) {
         g.cu_slots[x] = 0;

@@ -23,7 +23,7 @@ static int env_int(const char* name, int dflt, int lo, int hi) {
 // POTRF > TRSM > GEMM only), 308 ms (kind 3), 350 ms (kind 1).
 constexpr int MIXED_PRIO_KIND_DEFAULT = 2;
 static int mixed_prio_kind() {  // TTAD_MIXED_PRIO overrides it (tuning)
-    static const int k = env_int("TTAD_MIXED_PRIO", MIXED_PRIO_KIND_DEFAULT, 0, 3);
+    static const int k = env_int("TTAD_MIXED_PRIO", MIXED_PRIO_KIND_DEFAULT, 0, 4);
     return k;
 }
 

@@ -75,3 +75,29 @@ def test_dag_priorities_match_prio_kind():
                 assert p == exp
             if kind == 0:
                 assert p == {tt.TTAD_POTRF: 3.0, tt.TTAD_TRSM: 2.0}.get(k_, 1.0)
+
+
+@pytest.mark.parametrize("nb", [1, 2, 3, 6, 11])
+def test_dag_bottom_level_priorities(nb):
+    # prio_kind 4: longest path to the end of the DAG with weights POTRF 2, TRSM 2, GEMM/SYRK 1,
+    # checked against a dynamic program over the compiled edges in reverse topological order
+    tasks, indeg, prio, out_ptr, out_idx = tt.cholesky_dag(nb, 4)
+    n = len(tasks)
+    weight = {tt.TTAD_POTRF: 2.0, tt.TTAD_TRSM: 2.0, tt.TTAD_GEMM: 1.0, tt.TTAD_SYRK: 1.0}
+    pending = np.bincount(out_idx, minlength=n)
+    order = [t for t in range(n) if pending[t] == 0]
+    for t in order:  # grows while it is walked (Kahn)
+        for s in out_idx[out_ptr[t]:out_ptr[t + 1]]:
+            pending[s] -= 1
+            if pending[s] == 0:
+                order.append(s)
+    assert len(order) == n
+    bl = np.zeros(n)
+    for t in reversed(order):
+        succ = out_idx[out_ptr[t]:out_ptr[t + 1]]
+        bl[t] = weight[tasks[t][0]] + (bl[succ].max() if len(succ) else 0.0)
+    assert np.array_equal(bl, prio)
+    # the sink is POTRF(nb-1); the source POTRF(0) carries the longest path
+    names = [tt.task_name(t) for t in tasks]
+    assert prio[names.index(f"POTRF_{nb - 1}")] == 2.0
+    assert prio[names.index("POTRF_0")] == prio.max()
