@@ -709,6 +709,7 @@ static int graph_prepare(ttad_graph* g) {
         TTAD_HIP(g->d_q_head.alloc(g->n_buckets * 4));
         TTAD_HIP(g->d_q_tail.alloc(g->n_buckets * 4));
         TTAD_HIP(g->d_q_count.alloc(g->n_buckets * 4));
+        TTAD_HIP(g->d_xcd_run.alloc((size_t)n * NXCD * 4));
         TTAD_HIP(g->d_q_base.alloc((g->n_buckets + 1) * 4));
         TTAD_HIP(g->d_q_home.alloc(g->n_buckets * 4));
         TTAD_HIP(g->d_q_items.alloc((size_t)g->total_items * 8));
@@ -775,6 +776,7 @@ static DevGraph graph_image(ttad_graph* g, bool trace, int64_t spin_limit) {
     d.q_head = g->d_q_head.as<int32_t>();
     d.q_tail = g->d_q_tail.as<int32_t>();
     d.q_count = g->d_q_count.as<int32_t>();
+    d.xcd_run = g->d_xcd_run.as<int32_t>();
     d.q_base = g->d_q_base.as<int32_t>();
     d.q_home = g->d_q_home.as<int32_t>();
     d.q_items = g->d_q_items.as<int64_t>();

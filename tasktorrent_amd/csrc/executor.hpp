@@ -36,6 +36,7 @@ struct DevGraph {
     int32_t* cu_slots;          // [CU_KEYS] workers started per CU (slot 0 = primary)
     int32_t* idle_primary;      // primary workers currently looking for work
     int32_t* cu_crit;           // [CU_KEYS] critical items running per CU (its other worker waits)
+    int32_t* xcd_run;           // [n_tasks * NXCD] items of a task running on each XCD (release elision)
     int64_t spin_limit;         // watchdog in wall_clock64 ticks
 };
 

@@ -176,13 +176,23 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         const int task = (int)(e >> 20), item = (int)(e & 0xFFFFF);
         const DevTask t = g.tasks[task];
-        if (t.crit && tid == 0) __hip_atomic_fetch_add(g.cu_crit + my_key, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (tid == 0) {
+            if (t.crit) __hip_atomic_fetch_add(g.cu_crit + my_key, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(g.xcd_run + (long)task * NXCD + my_xcd, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         uint64_t t0 = 0;
         if (g.trace && tid == 0) t0 = wall_clock64();
         const bool ran = exec_item<VM>(t, item, g.mats, lds, g.abort_flag, g.spin_limit);
         __syncthreads();
         if (tid == 0) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            // Release elision: an agent-scope release writes back every dirty line of this XCD's L2,
+            // so the item that leaves no other item of its task running on this XCD releases for all
+            // of them (their stores completed before their own decrement, which came first); the
+            // others skip it. Each item decrements the task's remaining count after that, so the
+            // count reaches zero only after every XCD that ran items of the task has released.
+            if (__hip_atomic_fetch_add(g.xcd_run + (long)task * NXCD + my_xcd, -1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT) == 1)
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             if (!ran) __hip_atomic_store(g.abort_flag, 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (g.trace) {
                 const int slot = __hip_atomic_fetch_add(g.trace_cursor, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -194,8 +204,7 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
                     rec[3] = wall_clock64();
                 }
             }
-            // Counters are relaxed: the release fence above publishes this item's outputs (one L2 write-back
-            // per item; every release at agent scope writes back all dirty lines of the XCD's L2). The thread
+            // Counters are relaxed: the release fence above publishes this item's outputs. The thread
             // that takes a counter to zero acquires what the other decrements released and releases it again
             // before its own next counter update or queue entries (one acq_rel fence per task, and per
             // successor it makes ready).
@@ -228,6 +237,7 @@ __global__ void k_graph_reset_a(DevGraph g, int n_slots) {
         g.deps[x] = g.deps0[x];
         g.items_left[x] = g.tasks[x].n_items;
     }
+    for (long x = blockIdx.x * blockDim.x + threadIdx.x; x < (long)g.n_tasks * NXCD; x += stride) g.xcd_run[x] = 0;
     for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < g.n_buckets; x += stride) {
         g.q_head[x] = 0;
         g.q_tail[x] = g.tail0[x];
