@@ -71,7 +71,7 @@ struct ttad_context {
     bool ev_valid = false;
     double kernel_s = -1.0;  // event time of the last persistent factorization kernel (< 0: stream schedule)
     // solve-phase workspace (grown on demand)
-    ttad::DevBuf sw_Y, sw_D, sw_R, sw_flags, sw_ticket, sw_err, sw_norms;
+    ttad::DevBuf sw_Y, sw_D, sw_R, sw_flags, sw_ticket, sw_err, sw_norms, sw_part;
     // working copies of FP64 matrices for ttad_posv_mixed, keyed by (matrix id, storage)
     std::map<std::pair<long long, int>, ttad_matrix*> work_mats;
 };

@@ -163,15 +163,28 @@ static int residual(ttad_context* ctx, ttad_matrix* A, int nrhs, const double* B
         A->anorm_valid = true;
     }
     double worst = 0.0;
+    // column segments per row block: about 4096 workgroups in all
+    const int nseg = std::max(1, std::min(16, 4096 / std::max(1, t.P)));
+    const long n = A->n;
+    if ((rc = ensure(ctx->sw_part, (size_t)nseg * n * 32 * 8))) return rc;
+    double* part = ctx->sw_part.as<double>();
     for (int c0 = 0; c0 < nrhs; c0 += 32) {
         const int nc = std::min(32, nrhs - c0);
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(128), 0, s, norms, 3 * 32 + 8);
-        if (nc <= 16)
-            hipLaunchKernelGGL((k_residual<16>), dim3(t.P), dim3(NTHREADS), 0, s, Md, t, B + (long)c0 * ldb, ldb,
-                               X + (long)c0 * ldx, ldx, R + (long)c0 * ldr, ldr, nc, norms, 0);
-        else
-            hipLaunchKernelGGL((k_residual<32>), dim3(t.P), dim3(NTHREADS), 0, s, Md, t, B + (long)c0 * ldb, ldb,
-                               X + (long)c0 * ldx, ldx, R + (long)c0 * ldr, ldr, nc, norms, 0);
+        const double* Bc = B + (long)c0 * ldb;
+        const double* Xc = X + (long)c0 * ldx;
+        double* Rc = R + (long)c0 * ldr;
+        if (nc <= 16) {
+            hipLaunchKernelGGL((k_resid_part<16>), dim3(t.P * nseg), dim3(NTHREADS), 0, s, Md, t, Xc, ldx, part, n,
+                               nseg, nc);
+            hipLaunchKernelGGL((k_resid_reduce<16>), dim3(t.P), dim3(NTHREADS), 0, s, Md, t, part, n, nseg, Bc, ldb,
+                               Xc, ldx, Rc, ldr, nc, norms);
+        } else {
+            hipLaunchKernelGGL((k_resid_part<32>), dim3(t.P * nseg), dim3(NTHREADS), 0, s, Md, t, Xc, ldx, part, n,
+                               nseg, nc);
+            hipLaunchKernelGGL((k_resid_reduce<32>), dim3(t.P), dim3(NTHREADS), 0, s, Md, t, part, n, nseg, Bc, ldb,
+                               Xc, ldx, Rc, ldr, nc, norms);
+        }
         TTAD_HIP(hipGetLastError());
         unsigned long long h[3 * 32];
         TTAD_HIP(hipMemcpyAsync(h, norms, sizeof(unsigned long long) * 3 * nc, hipMemcpyDeviceToHost, s));

@@ -438,6 +438,90 @@ __device__ __forceinline__ void item_resid(const DevMat& M, const SweepTab& tab,
         }
 }
 
+// Split residual: workgroup (p, seg) sums A[p, q] x_q over the column blocks q of segment
+// seg and writes the 64 x nrhs partial to part[seg] (ld n); k_resid_reduce forms
+// r = b - sum of the partials and the per-rhs maxima. Many more workgroups than row blocks
+// keep enough loads in flight to stream A at HBM rate (one workgroup per row block, each
+// reading a whole block row of A, left most of the bandwidth unused).
+template <int NR>
+__global__ __launch_bounds__(NTHREADS) void k_resid_part(const DevMat* Mp, SweepTab tab, const double* X, long ldx,
+                                                         double* part, long n, int nseg, int nrhs) {
+    constexpr int NB = NR / 16;
+    const DevMat& M = *Mp;
+    const int P = tab.P;
+    const int p = blockIdx.x % P, seg = blockIdx.x / P;
+    const int q0 = (int)((long)P * seg / nseg), q1 = (int)((long)P * (seg + 1) / nseg);
+    const int g0 = blk_g0(M, tab, p), w = blk_rows(M, tab, p);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int tp = tab.blk_tile[p];
+    auto mode_of = [&](int q) {
+        const int tq = tab.blk_tile[q];
+        if (q == p) return 2;
+        if (tp > tq || (tp == tq && tab.blk_off[p] > tab.blk_off[q])) return 0;
+        return 1;
+    };
+    d4_t acc[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) acc[nb] = d4_t{0, 0, 0, 0};
+    double an[KS], bn[NB][KS];
+    if (q0 < q1) {
+        load_a<double>(an, M, tab, p, q0, mode_of(q0));
+        load_b<double, double, NR>(bn, X, ldx, blk_g0(M, tab, q0), blk_rows(M, tab, q0), nrhs);
+    }
+    for (int q = q0; q < q1; q++) {
+        double ac[KS], bc[NB][KS];
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            ac[ks] = an[ks];
+#pragma unroll
+            for (int nb = 0; nb < NB; nb++) bc[nb][ks] = bn[nb][ks];
+        }
+        if (q + 1 < q1) {
+            load_a<double>(an, M, tab, p, q + 1, mode_of(q + 1));
+            load_b<double, double, NR>(bn, X, ldx, blk_g0(M, tab, q + 1), blk_rows(M, tab, q + 1), nrhs);
+        }
+        reg_mma<double, double, NB>(acc, ac, bc);
+    }
+    TTAD_GLOBAL double* out = gmem(part) + (long)seg * n * NR;
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int m = wv * 16 + Mfma<double>::row(lane, r), c = nb * 16 + (lane & 15);
+            if (m < w && c < nrhs) out[(g0 + m) + (long)c * n] = acc[nb][r];
+        }
+}
+
+// r = b - sum over the segments' partials for one 64-row block, and the per-rhs maxima of
+// |r|, |x|, |b| (reduced in the workgroup, then one atomicMax per rhs and quantity).
+template <int NR>
+__global__ __launch_bounds__(NTHREADS) void k_resid_reduce(const DevMat* Mp, SweepTab tab, const double* part, long n,
+                                                           int nseg, const double* B, long ldb, const double* X,
+                                                           long ldx, double* R, long ldr, int nrhs,
+                                                           unsigned long long* norms) {
+    const int p = blockIdx.x;
+    const int g0 = blk_g0(*Mp, tab, p), w = blk_rows(*Mp, tab, p);
+    __shared__ unsigned long long s_max[3 * NR];
+    for (int e = threadIdx.x; e < 3 * NR; e += NTHREADS) s_max[e] = 0;
+    __syncthreads();
+    for (int e = threadIdx.x; e < SB * NR; e += NTHREADS) {
+        const int m = e % SB, c = e / SB;
+        if (m >= w || c >= nrhs) continue;
+        const long gi = g0 + m;
+        double sum = 0.0;
+        for (int sg = 0; sg < nseg; sg++) sum += gmem(part)[(long)sg * n * NR + gi + (long)c * n];
+        const double b = gmem(B)[gi + (long)c * ldb];
+        const double rv = b - sum;
+        gmem(R)[gi + (long)c * ldr] = rv;
+        atomicMax(s_max + 3 * c + 0, (unsigned long long)__double_as_longlong(fabs(rv)));
+        atomicMax(s_max + 3 * c + 1, (unsigned long long)__double_as_longlong(fabs(gmem(X)[gi + (long)c * ldx])));
+        atomicMax(s_max + 3 * c + 2, (unsigned long long)__double_as_longlong(fabs(b)));
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 3 * nrhs; e += NTHREADS)
+        if (s_max[e]) atomicMax(norms + e, s_max[e]);
+}
+
 template <int NR>
 __global__ __launch_bounds__(NTHREADS) void k_residual(const DevMat* Mp, SweepTab tab, const double* B, long ldb,
                                                        const double* X, long ldx, double* R, long ldr, int nrhs,
