@@ -86,12 +86,6 @@ static int64_t spin_ticks(ttad_context* ctx) {
     return (int64_t)rate_khz * 1000 * 5;
 }
 
-// Two sweep blocks per workgroup (k_sweep2, default); TTAD_SWEEP_PAIRS=0 runs one per workgroup.
-static bool sweep_pairs() {
-    static const bool on = env_int("TTAD_SWEEP_PAIRS", 1, 0, 1) != 0;
-    return on;
-}
-
 template <typename TL, typename TC>
 static void launch_sweep(ttad_matrix* L, int forward, const double* R, long ldr, TC* Y, long ldy, TC* D, long ldd,
                          double* X, long ldx, int nrhs, int32_t* flags, int32_t* ticket, int32_t* err, int64_t spin,
@@ -101,15 +95,6 @@ static void launch_sweep(ttad_matrix* L, int forward, const double* R, long ldr,
     hipLaunchKernelGGL(k_zero_i32, dim3((P + 255) / 256 + 1), dim3(256), 0, s, flags, P);
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, ticket, 1);
     const DevMat* Md = L->desc.as<DevMat>();
-    if (sweep_pairs()) {
-        if (nrhs <= 16)
-            hipLaunchKernelGGL((k_sweep2<TL, TC, 16>), dim3((P + 1) / 2), dim3(NTHREADS), 0, s, Md, t, forward, R, ldr,
-                               Y, ldy, D, ldd, X, ldx, nrhs, flags, ticket, err, spin);
-        else
-            hipLaunchKernelGGL((k_sweep2<TL, TC, 32>), dim3((P + 1) / 2), dim3(NTHREADS), 0, s, Md, t, forward, R, ldr,
-                               Y, ldy, D, ldd, X, ldx, nrhs, flags, ticket, err, spin);
-        return;
-    }
     if (nrhs <= 16)
         hipLaunchKernelGGL((k_sweep<TL, TC, 16>), dim3(P), dim3(NTHREADS), 0, s, Md, t, forward, R, ldr, Y, ldy, D,
                            ldd, X, ldx, nrhs, flags, ticket, err, spin);

@@ -243,53 +243,71 @@ __device__ __forceinline__ void item_sweep(const DevMat& M, const SweepTab& tab,
     int nwait = 0;
     const int need = forward ? p : P - 1 - p;  // predecessors, in sweep order
     int seen = 0;                               // predecessors known to be finished
-    TL an[KS];                                  // factor block of the next predecessor
-    TC bn[NB][KS];                              // its right-hand side block, when prefetched
-    bool have_b = false;
-    if (PF && need > 0) load_a<TL>(an, M, tab, p, forward ? 0 : P - 1, forward ? 0 : 1);
-    for (int i = 0; i < need; i++) {
-        const int q = forward ? i : P - 1 - i;
-        if (i >= seen) {  // uniform
-            const int left = need - i;
-            const bool far = left > tab.near;
-            const int want = far ? min(i + tab.batch, need - tab.near) : i + 1;
-            if (threadIdx.x == 0) s_seen[nwait & 1] = wait_count(flags, want, err, spin_limit, far);
-            __syncthreads();
-            seen = s_seen[nwait & 1];
-            nwait++;
+    const int mode = forward ? 0 : 1;
+    auto qof = [&](int i) { return forward ? i : P - 1 - i; };
+    // Waits (thread 0 polls, the barrier passes the count on) until predecessor i is final.
+    auto wait_for = [&](int i) {
+        if (i < seen) return;  // uniform
+        const int left = need - i;
+        const bool far = left > tab.near;
+        const int want = far ? min(i + tab.batch, need - tab.near) : i + 1;
+        if (threadIdx.x == 0) s_seen[nwait & 1] = wait_count(flags, want, err, spin_limit, far);
+        __syncthreads();
+        seen = s_seen[nwait & 1];
+        nwait++;
+    };
+    if constexpr (PF) {
+        // Two predecessors in flight: slot i & 1 holds predecessor i's factor block and, when it
+        // was known final at load time, its right-hand side block; the loads for i + 2 are issued
+        // before the product of i. A block behind the front is bound by these loads' latency,
+        // one round trip per two predecessors instead of one per predecessor.
+        TL a0[KS], a1[KS];
+        TC b0[NB][KS], b1[NB][KS];
+        bool h0 = false, h1 = false;
+        if (need > 0) load_a<TL>(a0, M, tab, p, qof(0), mode);
+        if (need > 1) load_a<TL>(a1, M, tab, p, qof(1), mode);
+        auto step = [&](int i, TL (&a)[KS], TC (&b)[NB][KS], bool& h) {
+            const int q = qof(i);
+            wait_for(i);
+            TC bc[NB][KS];
+            if (h) {
+#pragma unroll
+                for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+                    for (int ks = 0; ks < KS; ks++) bc[nb][ks] = b[nb][ks];
+            } else {
+                load_b<TC, TC, NR>(bc, Src, lds_, blk_g0(M, tab, q), blk_rows(M, tab, q), nrhs);
+            }
+            TL ac[KS];
+#pragma unroll
+            for (int ks = 0; ks < KS; ks++) ac[ks] = a[ks];
+            h = false;
+            if (i + 2 < need) {
+                const int qn = qof(i + 2);
+                load_a<TL>(a, M, tab, p, qn, mode);
+                if (i + 2 < seen) {
+                    load_b<TC, TC, NR>(b, Src, lds_, blk_g0(M, tab, qn), blk_rows(M, tab, qn), nrhs);
+                    h = true;
+                }
+            }
+            reg_mma<TC, TL, NB>(acc, ac, bc);
+        };
+        for (int i = 0; i < need; i += 2) {
+            step(i, a0, b0, h0);
+            if (i + 1 < need) step(i + 1, a1, b1, h1);
         }
-        TC bc[NB][KS];
-        if constexpr (!PF) {  // LDS-staged operands: few live registers (kernels with other paths)
+    } else {  // LDS-staged operands: few live registers (kernels with other paths)
+        for (int i = 0; i < need; i++) {
+            const int q = qof(i);
+            wait_for(i);
             TC* Ls = Ys + SB * LDY;
             TC* Bs = Ls + SB * LD;
-            fill_block<TL, TC>(Ls, M, tab, p, q, forward ? 0 : 1, false);
+            fill_block<TL, TC>(Ls, M, tab, p, q, mode, false);
             fill_rhs<TC, TC, NR>(Bs, Src, lds_, blk_g0(M, tab, q), blk_rows(M, tab, q), nrhs);
             __syncthreads();
             block_mma<TC, NR>(acc, Ls, Bs);
             __syncthreads();
-            continue;
         }
-        if (have_b) {
-#pragma unroll
-            for (int nb = 0; nb < NB; nb++)
-#pragma unroll
-                for (int ks = 0; ks < KS; ks++) bc[nb][ks] = bn[nb][ks];
-        } else {
-            load_b<TC, TC, NR>(bc, Src, lds_, blk_g0(M, tab, q), blk_rows(M, tab, q), nrhs);
-        }
-        TL ac[KS];
-#pragma unroll
-        for (int ks = 0; ks < KS; ks++) ac[ks] = an[ks];
-        have_b = false;
-        if (i + 1 < need) {
-            const int qn = forward ? i + 1 : P - 2 - i;
-            load_a<TL>(an, M, tab, p, qn, forward ? 0 : 1);
-            if (i + 1 < seen) {
-                load_b<TC, TC, NR>(bn, Src, lds_, blk_g0(M, tab, qn), blk_rows(M, tab, qn), nrhs);
-                have_b = true;
-            }
-        }
-        reg_mma<TC, TL, NB>(acc, ac, bc);
     }
     // t = rhs block - acc, staged in Ys as the B operand of the diagonal product
 #pragma unroll
@@ -328,204 +346,6 @@ __device__ __forceinline__ void item_sweep(const DevMat& M, const SweepTab& tab,
     if (threadIdx.x == 0) __hip_atomic_fetch_add(flags, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Diagonal step of a sweep block: y = Dinv (rhs - acc) (Dinv transposed for the backward
-// sweep, as staged in Ds), through Ys as the B operand; the result replaces acc. Starts with a
-// barrier (Ys free), ends with y in acc.
-template <typename TC, int NR>
-__device__ __forceinline__ void sweep_diag(typename Mfma<TC>::acc_t (&acc)[NR / 16], const TC (&rhs)[NR / 16][4],
-                                           const TC* Ds, TC* Ys) {
-    constexpr int NB = NR / 16, LD = SB + SPAD, LDY = NR + SPAD;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    __syncthreads();
-#pragma unroll
-    for (int nb = 0; nb < NB; nb++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int m = wv * 16 + Mfma<TC>::row(lane, r), n = nb * 16 + (lane & 15);
-            Ys[m * LDY + n] = rhs[nb][r] - acc[nb][r];
-        }
-    __syncthreads();
-#pragma unroll
-    for (int nb = 0; nb < NB; nb++) acc[nb] = typename Mfma<TC>::acc_t{0, 0, 0, 0};
-#pragma unroll
-    for (int ks = 0; ks < KS; ks++) {
-        const int kr = ks * 4 + (lane >> 4);
-        const TC a = Ds[kr * LD + wv * 16 + (lane & 15)];
-#pragma unroll
-        for (int nb = 0; nb < NB; nb++) acc[nb] = Mfma<TC>::mma(a, Ys[kr * LDY + nb * 16 + (lane & 15)], acc[nb]);
-    }
-}
-
-// Right-hand side block and diagonal inverse of sweep block p (Ds as item_sweep stages it).
-template <typename TC, int NR>
-__device__ __forceinline__ void sweep_block_inputs(TC (&rhs)[NR / 16][4], TC* Ds, const DevMat& M, const SweepTab& tab,
-                                                   int forward, int p, const double* R, long ldr, const TC* Y,
-                                                   long ldy, int nrhs) {
-    constexpr int NB = NR / 16, LD = SB + SPAD;
-    const int g0 = blk_g0(M, tab, p), w = blk_rows(M, tab, p);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int nb = 0; nb < NB; nb++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int m = wv * 16 + Mfma<TC>::row(lane, r), n = nb * 16 + (lane & 15);
-            rhs[nb][r] = TC(0);
-            if (m < w && n < nrhs)
-                rhs[nb][r] = forward ? (TC)gmem(R)[(g0 + m) + (long)n * ldr] : gmem(Y)[(g0 + m) + (long)n * ldy];
-        }
-    const TTAD_GLOBAL double* Dv = gmem(M.dinv[tab.blk_tile[p]]) + (long)(tab.blk_off[p] / SB) * SB * SB;
-    constexpr int PER = SB * SB / NTHREADS;
-    const int a = threadIdx.x % SB, b0 = threadIdx.x / SB;  // Dv[a + b*64] = Dinv(a, b)
-    double dv[PER];
-#pragma unroll
-    for (int q = 0; q < PER; q++) dv[q] = Dv[a + (b0 + q * (NTHREADS / SB)) * SB];
-#pragma unroll
-    for (int q = 0; q < PER; q++) {
-        const int b = b0 + q * (NTHREADS / SB);
-        if (forward) Ds[b * LD + a] = (TC)dv[q];  // (m, k) = Dinv(m, k)
-        else Ds[a * LD + b] = (TC)dv[q];          // (m, k) = Dinv(k, m)
-    }
-}
-
-// Store a solved block: Y[p] (forward) or D[p] and X[p] += D[p] (backward).
-template <typename TC, int NR>
-__device__ __forceinline__ void sweep_store(const typename Mfma<TC>::acc_t (&acc)[NR / 16], const DevMat& M,
-                                            const SweepTab& tab, int forward, int p, TC* Y, long ldy, TC* D, long ldd,
-                                            double* X, long ldx, int nrhs) {
-    const int g0 = blk_g0(M, tab, p), w = blk_rows(M, tab, p);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int nb = 0; nb < NR / 16; nb++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int m = wv * 16 + Mfma<TC>::row(lane, r), n = nb * 16 + (lane & 15);
-            if (m < w && n < nrhs) {
-                const long gi = g0 + m;
-                if (forward) {
-                    gmem(Y)[gi + (long)n * ldy] = acc[nb][r];
-                } else {
-                    gmem(D)[gi + (long)n * ldd] = acc[nb][r];
-                    gmem(X)[gi + (long)n * ldx] += (double)acc[nb][r];
-                }
-            }
-        }
-}
-
-// Two consecutive blocks of the sweep per workgroup, in sweep order pa then pb: the
-// predecessors of pa are shared, so each of their right-hand side blocks is loaded once for
-// both products, and the step from pa to pb stays inside the workgroup (no counter,
-// write-back or poll between them), which halves the dependent steps of the sweep chain.
-// The count in `flags` advances once per block as it is stored (pa before pb); blocks still
-// finish in sweep order, so it keeps its meaning for every waiter.
-template <typename TL, typename TC, int NR>
-__device__ __forceinline__ void item_sweep2(const DevMat& M, const SweepTab& tab, int forward, int t, const double* R,
-                                            long ldr, TC* Y, long ldy, TC* D, long ldd, double* X, long ldx, int nrhs,
-                                            int32_t* flags, int32_t* err, int64_t spin_limit, char* lds) {
-    constexpr int NB = NR / 16, LD = SB + SPAD, LDY = NR + SPAD;
-    const int P = tab.P;
-    const int ia = 2 * t, nown = min(2, P - ia);  // sweep-order indices ia (and ia + 1)
-    auto blk = [&](int i) { return forward ? i : P - 1 - i; };
-    const int pa = blk(ia), pb = blk(ia + nown - 1);
-    const int mode = forward ? 0 : 1;
-    TC* Ds0 = reinterpret_cast<TC*>(lds);  // Dinv of pa
-    TC* Ds1 = Ds0 + SB * LD;               // Dinv of pb
-    TC* Ys = Ds1 + SB * LD;                // B operand staging
-    const int lane = threadIdx.x & 63;
-    const TC* Src = forward ? Y : D;       // right-hand side blocks of the predecessors
-    const long lds_ = forward ? ldy : ldd;
-    typedef typename Mfma<TC>::acc_t acc_t;
-    acc_t acc0[NB], acc1[NB];
-#pragma unroll
-    for (int nb = 0; nb < NB; nb++) acc0[nb] = acc1[nb] = acc_t{0, 0, 0, 0};
-    TC rhs0[NB][4], rhs1[NB][4];
-    sweep_block_inputs<TC, NR>(rhs0, Ds0, M, tab, forward, pa, R, ldr, Y, ldy, nrhs);
-    if (nown > 1) sweep_block_inputs<TC, NR>(rhs1, Ds1, M, tab, forward, pb, R, ldr, Y, ldy, nrhs);
-
-    __shared__ int s_seen[2];
-    int nwait = 0, seen = 0;
-    const int need = ia;  // predecessors of pa
-    TL an0[KS], an1[KS];
-    TC bn[NB][KS];
-    bool have_b = false;
-    if (need > 0) {
-        load_a<TL>(an0, M, tab, pa, blk(0), mode);
-        if (nown > 1) load_a<TL>(an1, M, tab, pb, blk(0), mode);
-    }
-    for (int i = 0; i < need; i++) {
-        const int q = blk(i);
-        if (i >= seen) {  // uniform
-            const int left = need - i;
-            const bool far = left > tab.near;
-            const int want = far ? min(i + tab.batch, need - tab.near) : i + 1;
-            if (threadIdx.x == 0) s_seen[nwait & 1] = wait_count(flags, want, err, spin_limit, far);
-            __syncthreads();
-            seen = s_seen[nwait & 1];
-            nwait++;
-        }
-        TC bc[NB][KS];
-        if (have_b) {
-#pragma unroll
-            for (int nb = 0; nb < NB; nb++)
-#pragma unroll
-                for (int ks = 0; ks < KS; ks++) bc[nb][ks] = bn[nb][ks];
-        } else {
-            load_b<TC, TC, NR>(bc, Src, lds_, blk_g0(M, tab, q), blk_rows(M, tab, q), nrhs);
-        }
-        TL ac0[KS], ac1[KS];
-#pragma unroll
-        for (int ks = 0; ks < KS; ks++) {
-            ac0[ks] = an0[ks];
-            ac1[ks] = an1[ks];
-        }
-        have_b = false;
-        if (i + 1 < need) {
-            const int qn = blk(i + 1);
-            load_a<TL>(an0, M, tab, pa, qn, mode);
-            if (nown > 1) load_a<TL>(an1, M, tab, pb, qn, mode);
-            if (i + 1 < seen) {
-                load_b<TC, TC, NR>(bn, Src, lds_, blk_g0(M, tab, qn), blk_rows(M, tab, qn), nrhs);
-                have_b = true;
-            }
-        }
-        reg_mma<TC, TL, NB>(acc0, ac0, bc);
-        if (nown > 1) reg_mma<TC, TL, NB>(acc1, ac1, bc);
-    }
-    if (nown > 1) load_a<TL>(an1, M, tab, pb, pa, mode);  // the block coupling pb to pa
-    sweep_diag<TC, NR>(acc0, rhs0, Ds0, Ys);               // y(pa) in acc0
-    sweep_store<TC, NR>(acc0, M, tab, forward, pa, Y, ldy, D, ldd, X, ldx, nrhs);
-    __syncthreads();
-    // publish pa before pb: its consumers start on it while pb is solved
-    if (threadIdx.x == 0) __hip_atomic_fetch_add(flags, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    if (nown > 1) {
-        // acc1 += coupling block x y(pa), with y(pa) staged in Ys as the B operand
-        const int wv = threadIdx.x >> 6;
-#pragma unroll
-        for (int nb = 0; nb < NB; nb++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int m = wv * 16 + Mfma<TC>::row(lane, r), n = nb * 16 + (lane & 15);
-                Ys[m * LDY + n] = acc0[nb][r];
-            }
-        __syncthreads();
-        TC bc[NB][KS];
-        const int kg = lane >> 4;
-#pragma unroll
-        for (int nb = 0; nb < NB; nb++)
-#pragma unroll
-            for (int ks = 0; ks < KS; ks++) bc[nb][ks] = Ys[(4 * ks + kg) * LDY + nb * 16 + (lane & 15)];
-        reg_mma<TC, TL, NB>(acc1, an1, bc);
-        sweep_diag<TC, NR>(acc1, rhs1, Ds1, Ys);
-        sweep_store<TC, NR>(acc1, M, tab, forward, pb, Y, ldy, D, ldd, X, ldx, nrhs);
-        __syncthreads();
-        if (threadIdx.x == 0) __hip_atomic_fetch_add(flags, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
-}
-
-template <typename TC, int NR>
-constexpr int sweep2_lds_bytes() {
-    return (2 * SB * (SB + SPAD) + SB * (NR + SPAD)) * (int)sizeof(TC);
-}
-
 // LDS of item_sweep<.., PF> (and of item_resid<NR, false> for TC = double).
 template <typename TC, int NR, bool PF = true>
 constexpr int sweep_lds_bytes() {
@@ -547,21 +367,6 @@ __global__ __launch_bounds__(NTHREADS) void k_sweep(const DevMat* Mp, SweepTab t
     if (t >= tab.P) return;
     const int p = forward ? t : tab.P - 1 - t;
     item_sweep<TL, TC, NR>(*Mp, tab, forward, p, R, ldr, Y, ldy, D, ldd, X, ldx, nrhs, flags, err, spin_limit, lds);
-}
-
-// Two blocks per workgroup (item_sweep2); launched with ceil(P / 2) workgroups.
-template <typename TL, typename TC, int NR>
-__global__ __launch_bounds__(NTHREADS) void k_sweep2(const DevMat* Mp, SweepTab tab, int forward, const double* R,
-                                                     long ldr, TC* Y, long ldy, TC* D, long ldd, double* X, long ldx,
-                                                     int nrhs, int32_t* flags, int32_t* ticket, int32_t* err,
-                                                     int64_t spin_limit) {
-    __shared__ __align__(16) char lds[sweep2_lds_bytes<TC, NR>()];
-    __shared__ int s_t;
-    if (threadIdx.x == 0) s_t = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const int t = s_t;
-    if (2 * t >= tab.P) return;
-    item_sweep2<TL, TC, NR>(*Mp, tab, forward, t, R, ldr, Y, ldy, D, ldd, X, ldx, nrhs, flags, err, spin_limit, lds);
 }
 
 // Residual r = b - A x (FP64, A symmetric from its lower tiles) for one 64-row block p,
