@@ -325,7 +325,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{workload}: Cholesky + FP64 refinement to eta<=1e-12, n={n} b={b} nrhs={nrhs}, "
                                f"A=D(BB^T/n+1e-2 I)D",
-                   "n": n, "tile": b, "nrhs": nrhs, "prio_kind": args.prio, "schedule": "persistent"},
+                   "n": n, "tile": b, "nrhs": nrhs, "prio_kind": args.prio, "mixed_prio_kind": int(os.environ.get("TTAD_MIXED_PRIO", "4")), "schedule": "persistent"},
         "legs": res,
         "roofline": {"bound": "mfma", "achieved": round(F / t_fac64 / 1e12, 3), "peak": PEAK["f64"],
                      "unit": "TFLOP/s", "frac": round(F / t_fac64 / 1e12 / PEAK["f64"], 4),

@@ -17,11 +17,12 @@ static int env_int(const char* name, int dflt, int lo, int hi) {
     return std::min(hi, std::max(lo, atoi(v)));
 }
 
-// Priority kind of the factorizations inside ttad_posv_mixed (2d_cholesky.cpp `--kind`):
-// 2, critical path first (every task of panel column j at 9(nb-j)-c). Measured at n = 32768
-// with two workers per CU: bf16x3 time-to-solution 293 ms (kind 2) vs 335 ms (kind 0,
-// POTRF > TRSM > GEMM only), 308 ms (kind 3), 350 ms (kind 1).
-constexpr int MIXED_PRIO_KIND_DEFAULT = 2;
+// Priority kind of the factorizations inside ttad_posv_mixed: 4, the bottom level of each
+// task in the DAG (its longest remaining path). The bf16x3 factorization is short enough for
+// its tail, where the serial update chains of the last tiles meet the panel chain, to matter.
+// Measured at n = 32768: time to solution 131 ms (kind 4) vs 133 ms (kind 3) and 140 ms (kind 2,
+// the reference's critical-path priorities).
+constexpr int MIXED_PRIO_KIND_DEFAULT = 4;
 static int mixed_prio_kind() {  // TTAD_MIXED_PRIO overrides it (tuning)
     static const int k = env_int("TTAD_MIXED_PRIO", MIXED_PRIO_KIND_DEFAULT, 0, 4);
     return k;
