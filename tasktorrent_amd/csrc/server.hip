@@ -22,23 +22,20 @@ namespace ttad {
 constexpr int SRV_LDS = cmax(LDS_ITEM_MAX, sweep_lds_bytes<double, 32, false>());
 
 __device__ __forceinline__ int64_t srv_try_pop(const DevServer& s, uint32_t& rng) {
-    // lane 0 only: claim the head of the first non-empty ring in priority order: a fetch-add
-    // when the ring holds more entries than there are workers (the claim cannot pass the
-    // tail), else a CAS whose losers back off a random interval
-    const unsigned long long margin = 2ull * gridDim.x;
+    // lane 0 only: claim the head of the first non-empty ring in priority order. A claim takes
+    // one credit from the ring's count of unclaimed entries (a claimer that finds none gives it
+    // back and rescans after a random back-off), then its slot with a fetch-add on the head:
+    // credits never exceed the pushed entries, so the slot never passes the tail.
     for (int b = 0; b < s.n_buckets; b++) {
         const SrvRing& R = s.rings[b];
-        unsigned long long h = __hip_atomic_load(R.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned long long t = __hip_atomic_load(R.tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (h >= t) continue;
-        if (t - h > margin) {
-            h = __hip_atomic_fetch_add(R.head, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else if (!__hip_atomic_compare_exchange_strong(R.head, &h, h + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_AGENT)) {
+        if (__hip_atomic_load(R.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= 0) continue;
+        if (__hip_atomic_fetch_add(R.count, -1ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= 0) {
+            __hip_atomic_fetch_add(R.count, 1ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             rng = rng * 1664525u + 1013904223u;
             for (uint32_t q = rng >> 28; q > 0; q--) __builtin_amdgcn_s_sleep(1);
-            return -1;  // lost the race: caller rescans
+            return -1;  // lost the race for the last credits: caller rescans
         }
+        const unsigned long long h = __hip_atomic_fetch_add(R.head, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned long long cap = 1ull << s.ring_log2;
         const unsigned long long lap = ((h >> s.ring_log2) + 1) & 0xFFFFFFull;
         const unsigned long long* slot = R.slots + (h & (cap - 1));
@@ -512,9 +509,9 @@ int ttad_server_create(ttad_context* ctx, const ttad_server_config* cfg_in, ttad
     const int lg = 21;  // ring capacity per bucket; admission keeps admitted items below it
     S->ring_log2 = lg;
     S->items_cap = 1LL << lg;
-    TTAD_HIP(S->ring_hdr.alloc(RING_BUCKETS * 2 * 8));
+    TTAD_HIP(S->ring_hdr.alloc(RING_BUCKETS * 3 * 8));
     TTAD_HIP(S->ring_slots.alloc((size_t)RING_BUCKETS << lg << 3));
-    TTAD_HIP(hipMemset(S->ring_hdr.p, 0, RING_BUCKETS * 2 * 8));
+    TTAD_HIP(hipMemset(S->ring_hdr.p, 0, RING_BUCKETS * 3 * 8));
     TTAD_HIP(hipMemset(S->ring_slots.p, 0, (size_t)RING_BUCKETS << lg << 3));
     TTAD_HIP(S->misc.alloc(256));
     TTAD_HIP(hipMemset(S->misc.p, 0, 256));
@@ -546,8 +543,9 @@ int ttad_server_create(ttad_context* ctx, const ttad_server_config* cfg_in, ttad
     d.ring_log2 = lg;
     unsigned long long* hdr = S->ring_hdr.as<unsigned long long>();
     for (int b = 0; b < RING_BUCKETS; b++) {
-        d.rings[b].head = hdr + 2 * b;
-        d.rings[b].tail = hdr + 2 * b + 1;
+        d.rings[b].head = hdr + 3 * b;
+        d.rings[b].tail = hdr + 3 * b + 1;
+        d.rings[b].count = (long long*)(hdr + 3 * b + 2);
         d.rings[b].slots = S->ring_slots.as<unsigned long long>() + ((size_t)b << lg);
     }
     d.tasks = S->tasks.as<SrvTask>();

@@ -77,6 +77,8 @@ struct SrvRing {                // per bucket ring of work-item entries
     unsigned long long* head;   // claimed
     unsigned long long* tail;   // reserved
     unsigned long long* slots;  // entry = ((lap+1) & 0xFFFFFF) << 40 | task << 16 | item
+    long long* count;           // entries pushed and not yet claimed (claim credits; dips below zero while a
+                                // failed claim gives its credit back)
 };
 
 struct DevServer {
@@ -114,6 +116,7 @@ __device__ __forceinline__ void srv_push(const DevServer& s, int task) {
         const unsigned long long e = (lap << 40) | ((unsigned long long)task << 16) | (unsigned long long)r;
         __hip_atomic_store(R.slots + (idx & (cap - 1)), e, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
+    __hip_atomic_fetch_add(R.count, (long long)t.n_items, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // A task finished: decrement successors, push those that become ready.
