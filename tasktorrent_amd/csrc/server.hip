@@ -358,7 +358,7 @@ static std::shared_ptr<Template> build_template(const std::vector<int>& bsz, int
         for (size_t q = 0; q < desc.size(); q++) {
             const ttad_task_desc& d = desc[q];
             const int kind = d.kind;
-            const int items = items_of(kind, bsz[d.i], bsz[d.j]);
+            const int items = kind == K_POTRF ? potrf_row_items(bsz[d.j]) : items_of(kind, bsz[d.i], bsz[d.j]);
             const int dp = (kind == K_GEMM || kind == K_SYRK) ? prec : (prec == P_F64 ? P_F64 : P_F32);
             const int id = add(kind, items, d.i, d.j, d.k, indeg[q]);
             T->tasks[id].prec = dp;
@@ -442,6 +442,8 @@ static FactorLayout factor_layout(const std::vector<int>& bsz, int storage, std:
     o = align_up(o + nt * 8, 256);
     const long long o_info = o;
     o = align_up(o + 8, 256);
+    const long long o_pflags = o;  // row-block POTRF progress flags (uploaded as zeros with the tables)
+    o = align_up(o + (long long)nt * POTRF_FLAGS * 4, 256);
     const long long total = o;
     // host image of the tables (bsz, boff, tile pointers, dinv pointers, info)
     host_tables.assign((size_t)(total - o_bsz), 0);
@@ -471,6 +473,7 @@ static FactorLayout factor_layout(const std::vector<int>& bsz, int storage, std:
     d.tiles = (void* const*)dev(o_tab);
     d.dinv = (double* const*)dev(o_dtab);
     d.info = (int32_t*)dev(o_info);
+    d.pflags = (int32_t*)dev(o_pflags);
     return FactorLayout{total, o_bsz};
 }
 

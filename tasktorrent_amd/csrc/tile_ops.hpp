@@ -1015,6 +1015,7 @@ constexpr int LDS_TRSM = cmax(GemmNT<double, double, double, TRSM_BM, DB, 32, tr
                               GemmNT<double, double, double, TRSM_BM, DB, DB, false, true>::STAGE * 8);
 constexpr int LDS_TRSM_F32 = cmax(GemmNT<float, float, float, TRSM_BM, DB, 32, true, true>::LDS_BYTES,
                                   GemmNT<float, float, float, TRSM_BM, DB, DB, false, true>::STAGE * 4);
-constexpr int LDS_ITEM_MAX = cmax(cmax(cmax(LDS_GEMM_F64, LDS_GEMM_F32), cmax(LDS_GEMM_BF16, LDS_POTRF)), LDS_TRSM);
+constexpr int LDS_ITEM_MAX =
+    cmax(cmax(cmax(LDS_GEMM_F64, LDS_GEMM_F32), cmax(LDS_GEMM_BF16, LDS_POTRF)), cmax(LDS_TRSM, LDS_POTRF_ROWS));
 
 }  // namespace ttad
