@@ -11,7 +11,7 @@ enum Prec : int32_t { P_F64 = 0, P_F32 = 1, P_BF16 = 2, P_BF16X3 = 3 };
 
 constexpr int NTHREADS = 256;  // one workgroup = 4 wave64
 constexpr int GEMM_BM = 128;   // GEMM/SYRK work item = 128x128 output sub-tile
-constexpr int TRSM_BM = 64;    // TRSM work item = 64-row block of the tile (its latency is on the critical path)
+constexpr int TRSM_BM = 32;    // TRSM work item = 32-row block of the tile (its latency is on the critical path)
 constexpr int DB = 64;         // diagonal sub-block size whose inverse POTRF leaves for TRSM
 
 // One tiled symmetric matrix in HBM. Tile (i,j), i >= j, is a column-major
