@@ -45,6 +45,10 @@ extern "C" {
 #define TTAD_TRSM 1
 #define TTAD_SYRK 2
 #define TTAD_GEMM 3
+/* No tile access: the workgroup spins for k microseconds (scheduler-overhead benchmark,
+ * replaces the spin_for bodies of miniapp/mini_benchmarks/shared/ttor_cholesky.cpp:22-28);
+ * mat is ignored and no matrix needs to be bound; i only places the task's queue (as for GEMM). */
+#define TTAD_SPIN 5
 
 /* ---- schedules of ttad_potrf_tiled ------------------------------------------ */
 #define TTAD_SCHED_STREAM 0      /* one launch per panel step (bulk-synchronous) */
@@ -132,7 +136,7 @@ int ttad_last_elapsed(ttad_context* ctx, double* seconds);
 
 /* ---- device task graphs: the Taskflow<K>/Threadpool drop-in ------------------- */
 typedef struct {
-    int32_t kind;  /* TTAD_POTRF .. TTAD_GEMM */
+    int32_t kind;  /* TTAD_POTRF .. TTAD_GEMM, or TTAD_SPIN */
     int32_t prec;  /* TTAD_F64 / TTAD_F32 / TTAD_BF16 / TTAD_BF16X3 */
     int32_t mat;   /* matrix slot bound with ttad_graph_bind */
     int32_t i, j, k;

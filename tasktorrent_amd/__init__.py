@@ -17,11 +17,11 @@ import numpy as np
 
 from . import _capi
 from ._capi import (TTAD_BF16, TTAD_BF16X3, TTAD_F32, TTAD_F64, TTAD_GEMM, TTAD_POTRF, TTAD_SCHED_PERSISTENT,
-                    TTAD_SCHED_STREAM, TTAD_SYRK, TTAD_TRSM, TtadError, lib, lib_path)
+                    TTAD_SCHED_STREAM, TTAD_SPIN, TTAD_SYRK, TTAD_TRSM, TtadError, lib, lib_path)
 
 __all__ = [
     "Context", "TiledMatrix", "Graph", "Server", "TtadError", "lib", "lib_path",
-    "TTAD_F64", "TTAD_F32", "TTAD_BF16", "TTAD_BF16X3", "TTAD_POTRF", "TTAD_TRSM", "TTAD_SYRK", "TTAD_GEMM",
+    "TTAD_F64", "TTAD_F32", "TTAD_BF16", "TTAD_BF16X3", "TTAD_POTRF", "TTAD_TRSM", "TTAD_SYRK", "TTAD_GEMM", "TTAD_SPIN",
     "TTAD_SCHED_STREAM", "TTAD_SCHED_PERSISTENT",
 ]
 
@@ -364,6 +364,32 @@ def cholesky_dag(num_blocks: int, prio_kind: int = 2):
                                         ip(out_idx)))
     tl = [(t.kind, t.prec, t.mat, t.i, t.j, t.k) for t in tasks]
     return tl, indeg, prio, out_ptr, out_idx[:e]
+
+
+def cholesky_spin_bench(ctx: Context, num_blocks: int, spin_us: int, prio_kind: int = 2, n_workers: int = 0,
+                        reps: int = 3) -> dict:
+    """Scheduler-overhead benchmark (mini_benchmarks/shared/ttor_cholesky.cpp:22-108): the Cholesky DAG of
+    num_blocks x num_blocks tiles with every task body a spin of spin_us microseconds on the device executor.
+    efficiency = tasks * spin / (elapsed * workers), as the reference computes it (:104-108); the best of reps.
+    """
+    tasks, indeg, prio, out_ptr, out_idx = cholesky_dag(num_blocks, prio_kind)
+    spin = [(TTAD_SPIN, 0, 0, t[3], t[4], int(spin_us)) for t in tasks]  # keeps (i, j): queue homing
+    G = Graph(ctx, spin, indeg, out_ptr, out_idx, prio)
+    try:
+        G.seed([task_name(t) for t in tasks].index("POTRF_0"))
+        best = None
+        for _ in range(reps):
+            G.launch(n_workers)
+            rc, st = G.wait(120.0)
+            _capi.check(rc)
+            if best is None or st["elapsed_s"] < best["elapsed_s"]:
+                best = st
+    finally:
+        G.close()
+    n = len(tasks)
+    eff = n * spin_us * 1e-6 / (best["elapsed_s"] * best["workers"])
+    return {"tasks": n, "spin_us": spin_us, "workers": best["workers"], "elapsed_s": best["elapsed_s"],
+            "efficiency": eff}
 
 
 def task_name(t) -> str:

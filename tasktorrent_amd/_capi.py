@@ -22,6 +22,7 @@ TTAD_ERR_NOT_FOUND = 9
 
 TTAD_F64, TTAD_F32, TTAD_BF16, TTAD_BF16X3 = 0, 1, 2, 3
 TTAD_POTRF, TTAD_TRSM, TTAD_SYRK, TTAD_GEMM = 0, 1, 2, 3
+TTAD_SPIN = 5
 TTAD_SCHED_STREAM, TTAD_SCHED_PERSISTENT = 0, 1
 
 _NAMES = {0: "OK", 1: "INVALID", 2: "OOM", 3: "NOT_SPD", 4: "TIMEOUT", 5: "DEVICE", 6: "IR_STALLED", 7: "GRAPH",

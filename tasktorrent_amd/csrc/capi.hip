@@ -578,6 +578,7 @@ int ttad_graph_create(ttad_context* ctx, int n_tasks, const ttad_task_desc* task
     g->ctx = ctx;
     g->n_tasks = n_tasks;
     g->tasks.resize(n_tasks);
+    g->spin_us.assign(n_tasks, 0);
     g->indegree.assign(indegree, indegree + n_tasks);
     g->out_ptr.assign(out_ptr, out_ptr + n_tasks + 1);
     if (g->out_ptr[0] != 0) return set_error(TTAD_ERR_INVALID, "graph_create: out_ptr[0] != 0");
@@ -588,7 +589,8 @@ int ttad_graph_create(ttad_context* ctx, int n_tasks, const ttad_task_desc* task
         if (g->out_ptr[t + 1] < g->out_ptr[t]) return set_error(TTAD_ERR_INVALID, "graph_create: out_ptr not sorted");
         if (indegree[t] < 1) return set_error(TTAD_ERR_GRAPH, "graph_create: task %d has indegree < 1", t);
         const ttad_task_desc& d = tasks[t];
-        if (d.kind < TTAD_POTRF || d.kind > TTAD_GEMM || d.prec < 0 || d.prec > 3 || d.mat < 0 || d.mat > 65535)
+        if (((d.kind < TTAD_POTRF || d.kind > TTAD_GEMM) && d.kind != TTAD_SPIN) || d.prec < 0 || d.prec > 3 ||
+            d.mat < 0 || d.mat > 65535 || (d.kind == TTAD_SPIN && (d.k < 0 || d.k > 10000000)))
             return set_error(TTAD_ERR_INVALID, "graph_create: bad descriptor for task %d", t);
         DevTask& x = g->tasks[t];
         x.kind = d.kind;
@@ -600,6 +602,7 @@ int ttad_graph_create(ttad_context* ctx, int n_tasks, const ttad_task_desc* task
         x.n_items = 1;
         x.bucket = 0;
         x.crit = 0;
+        g->spin_us[t] = d.kind == TTAD_SPIN ? d.k : 0;
     }
     for (int e = 0; e < n_edges; e++)
         if (g->out_idx[e] < 0 || g->out_idx[e] >= n_tasks)
@@ -622,7 +625,8 @@ int ttad_graph_create(ttad_context* ctx, int n_tasks, const ttad_task_desc* task
         const DevTask& x = g->tasks[t];
         // GEMM/SYRK items are homed by output tile row: the items of one update share their
         // A and B panels, and the updates of one row share A, in one XCD's L2
-        if (xcd_affinity() && (x.kind == K_GEMM || x.kind == K_SYRK)) home[t] = x.i % NXCD;
+        // spin tasks are homed like GEMM/SYRK (by their row index), so that tiny tasks spread their claims
+        if (xcd_affinity() && (x.kind == K_GEMM || x.kind == K_SYRK || x.kind == K_NOP)) home[t] = x.i % NXCD;
     }
     // work queues in priority order; within a bucket the shared queue, then one per home XCD
     std::vector<int> qid((size_t)n_prio * (NXCD + 1), -1);
@@ -671,8 +675,16 @@ int ttad_graph_clear_seeds(ttad_graph* g) {
 static int graph_prepare(ttad_graph* g) {
     const int n = g->n_tasks;
     if (g->mats_dirty) {
+        int rate_khz = 100000;
+        (void)hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, g->ctx->device);
         for (int t = 0; t < n; t++) {
             DevTask& x = g->tasks[t];
+            if (x.kind == K_NOP) {  // spin task: k microseconds, stored as device clock ticks
+                x.n_items = 1;
+                x.mat = 0;
+                x.k = (int32_t)std::min<long long>((long long)g->spin_us[t] * rate_khz / 1000, INT32_MAX);
+                continue;
+            }
             if (x.mat >= (int)g->mats.size() || !g->mats[x.mat])
                 return set_error(TTAD_ERR_INVALID, "graph: task %d uses unbound matrix slot %d", t, x.mat);
             ttad_matrix* M = g->mats[x.mat];
@@ -685,7 +697,8 @@ static int graph_prepare(ttad_graph* g) {
                                                                   : std::max(1, items_of(x.kind, M->bsz[x.i], M->bsz[x.j]));
         }
         int needed = 0;
-        for (auto& x : g->tasks) needed |= item_variant(x.kind, x.prec, g->mats[x.mat]->storage);
+        for (auto& x : g->tasks)
+            if (x.kind != K_NOP) needed |= item_variant(x.kind, x.prec, g->mats[x.mat]->storage);
         g->vset = vset_index(needed);
         g->bucket_items.assign(g->n_buckets, 0);
         g->total_items = 0;

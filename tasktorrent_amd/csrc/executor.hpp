@@ -146,6 +146,14 @@ __device__ __attribute__((noinline)) void trsm_call(const TS* L, long ldl, const
 template <int VM>
 __device__ __forceinline__ bool exec_item(const DevTask& t, int it, const DevMat* mats, char* lds, int32_t* err,
                                           int64_t spin_limit) {
+    if (t.kind == K_NOP) {  // spin task (scheduler-overhead benchmark): k device clock ticks, no tile access
+        if (threadIdx.x == 0) {
+            const uint64_t t0 = wall_clock64();
+            while (wall_clock64() - t0 < (uint64_t)t.k) __builtin_amdgcn_s_sleep(1);
+        }
+        __syncthreads();
+        return true;
+    }
     const DevMat& M = mats[t.mat];
     const int nt = M.nt;
     auto tile = [&](int i, int j) { return M.tiles[i + (long)j * nt]; };

@@ -100,6 +100,7 @@ struct ttad_graph {
     std::vector<int32_t> indegree, seeds, out_ptr, out_idx, q_base, bucket_items;
     std::vector<int32_t> q_home;  // per queue: -1 shared, else the XCD whose workers it feeds first
     std::vector<ttad_matrix*> mats;
+    std::vector<int32_t> spin_us;  // per task: spin time of a TTAD_SPIN task (microseconds)
     bool seeds_dirty = true, mats_dirty = true, running = false;
     ttad::DevBuf d_tasks, d_out_ptr, d_out_idx, d_deps0, d_tail0, d_init, d_deps, d_items_left, d_q_head, d_q_tail, d_q_count, d_xcd_run,
         d_q_base, d_q_home, d_q_items, d_done, d_abort, d_mats, d_trace, d_trace_cursor, d_cu_slots;

@@ -121,3 +121,36 @@ def test_mixed_factor_close(ctx, oracle, prec):
     assert rel(L, Lref) < tol
     L2, _ = factor(ctx, A, [b] * 4, tt.TTAD_SCHED_STREAM, prec=prec)
     assert np.array_equal(L, L2)
+
+
+def _longest_path_tasks(indeg_edges, n):
+    out_ptr, out_idx = indeg_edges
+    pending = np.bincount(out_idx, minlength=n)
+    order = [t for t in range(n) if pending[t] == 0]
+    for t in order:
+        for s in out_idx[out_ptr[t]:out_ptr[t + 1]]:
+            pending[s] -= 1
+            if pending[s] == 0:
+                order.append(s)
+    bl = np.zeros(n)
+    for t in reversed(order):
+        succ = out_idx[out_ptr[t]:out_ptr[t + 1]]
+        bl[t] = 1 + (bl[succ].max() if len(succ) else 0)
+    return int(bl.max())
+
+
+@pytest.mark.parametrize("nb,workers", [(8, 16), (12, 0)])
+def test_spin_bench_scheduler_overhead(ctx, nb, workers):
+    # mini_benchmarks/shared/ttor_cholesky.cpp: the Cholesky DAG with spin bodies. The run can be
+    # no shorter than the longest chain of spins nor than the total spin spread over the workers,
+    # and the reported efficiency follows the reference's formula.
+    spin_us = 200
+    tasks, indeg, prio, out_ptr, out_idx = tt.cholesky_dag(nb, 2)
+    crit = _longest_path_tasks((out_ptr, out_idx), len(tasks))
+    r = tt.cholesky_spin_bench(ctx, nb, spin_us, n_workers=workers, reps=2)
+    assert r["tasks"] == len(tasks)
+    lower = max(crit * spin_us * 1e-6, len(tasks) * spin_us * 1e-6 / r["workers"])
+    assert r["elapsed_s"] >= 0.95 * lower
+    assert r["elapsed_s"] <= 2.0 * lower + 5e-3
+    assert 0.0 < r["efficiency"] <= 1.0
+    assert r["efficiency"] == pytest.approx(len(tasks) * spin_us * 1e-6 / (r["elapsed_s"] * r["workers"]))
