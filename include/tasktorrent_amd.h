@@ -49,6 +49,14 @@ extern "C" {
  * replaces the spin_for bodies of miniapp/mini_benchmarks/shared/ttor_cholesky.cpp:22-28);
  * mat is ignored and no matrix needs to be bound; i only places the task's queue (as for GEMM). */
 #define TTAD_SPIN 5
+/* GEMM / SYRK in accumulate mode (2d_cholesky.cpp --accumulate, :149-155, 479-608): the product
+ * is formed from zero and added into the C tile under a lock per 128x128 sub-tile, in the
+ * order the tasks finish (the reference's ACCU task, fused into the product). */
+#define TTAD_GEMM_ACC 6
+#define TTAD_SYRK_ACC 7
+/* OR into prio_kind of ttad_potrf_tiled / ttad_cholesky_dag: the accumulate-mode DAG (GEMM and
+ * SYRK tasks of one tile run in any order; POTRF(j) waits for j of them, TRSM(i,j) for 1 + j). */
+#define TTAD_ACCUMULATE 0x100
 
 /* ---- schedules of ttad_potrf_tiled ------------------------------------------ */
 #define TTAD_SCHED_STREAM 0      /* one launch per panel step (bulk-synchronous) */

@@ -16,12 +16,14 @@ from pathlib import Path
 import numpy as np
 
 from . import _capi
-from ._capi import (TTAD_BF16, TTAD_BF16X3, TTAD_F32, TTAD_F64, TTAD_GEMM, TTAD_POTRF, TTAD_SCHED_PERSISTENT,
-                    TTAD_SCHED_STREAM, TTAD_SPIN, TTAD_SYRK, TTAD_TRSM, TtadError, lib, lib_path)
+from ._capi import (TTAD_ACCUMULATE, TTAD_BF16, TTAD_BF16X3, TTAD_F32, TTAD_F64, TTAD_GEMM, TTAD_GEMM_ACC,
+                    TTAD_POTRF, TTAD_SCHED_PERSISTENT, TTAD_SCHED_STREAM, TTAD_SPIN, TTAD_SYRK, TTAD_SYRK_ACC,
+                    TTAD_TRSM, TtadError, lib, lib_path)
 
 __all__ = [
     "Context", "TiledMatrix", "Graph", "Server", "TtadError", "lib", "lib_path",
     "TTAD_F64", "TTAD_F32", "TTAD_BF16", "TTAD_BF16X3", "TTAD_POTRF", "TTAD_TRSM", "TTAD_SYRK", "TTAD_GEMM", "TTAD_SPIN",
+    "TTAD_GEMM_ACC", "TTAD_SYRK_ACC", "TTAD_ACCUMULATE",
     "TTAD_SCHED_STREAM", "TTAD_SCHED_PERSISTENT",
 ]
 

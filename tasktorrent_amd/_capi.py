@@ -23,6 +23,8 @@ TTAD_ERR_NOT_FOUND = 9
 TTAD_F64, TTAD_F32, TTAD_BF16, TTAD_BF16X3 = 0, 1, 2, 3
 TTAD_POTRF, TTAD_TRSM, TTAD_SYRK, TTAD_GEMM = 0, 1, 2, 3
 TTAD_SPIN = 5
+TTAD_GEMM_ACC, TTAD_SYRK_ACC = 6, 7  # accumulate-mode GEMM / SYRK (product added into C under a sub-tile lock)
+TTAD_ACCUMULATE = 0x100  # OR into prio_kind: the accumulate-mode DAG
 TTAD_SCHED_STREAM, TTAD_SCHED_PERSISTENT = 0, 1
 
 _NAMES = {0: "OK", 1: "INVALID", 2: "OOM", 3: "NOT_SPD", 4: "TIMEOUT", 5: "DEVICE", 6: "IR_STALLED", 7: "GRAPH",

@@ -327,6 +327,7 @@ int ttad_matrix_create(ttad_context* ctx, int num_blocks, const int* block_sizes
     TTAD_HIP(M->tab_dinv.alloc(nt * sizeof(double*)));
     TTAD_HIP(M->info.alloc(sizeof(int)));
     TTAD_HIP(M->pflags.alloc((size_t)nt * POTRF_FLAGS * sizeof(int32_t)));
+    TTAD_HIP(M->locks.alloc((size_t)nt * nt * 16 * sizeof(int32_t)));
     TTAD_HIP(M->desc.alloc(sizeof(DevMat)));
     TTAD_HIP(hipMemcpy(M->tab_bsz.p, M->bsz.data(), nt * 4, hipMemcpyHostToDevice));
     TTAD_HIP(hipMemcpy(M->tab_boff.p, M->boff.data(), (nt + 1) * 4, hipMemcpyHostToDevice));
@@ -334,6 +335,7 @@ int ttad_matrix_create(ttad_context* ctx, int num_blocks, const int* block_sizes
     TTAD_HIP(hipMemcpy(M->tab_dinv.p, dp.data(), dp.size() * sizeof(double*), hipMemcpyHostToDevice));
     TTAD_HIP(hipMemset(M->info.p, 0, sizeof(int)));
     TTAD_HIP(hipMemset(M->pflags.p, 0, (size_t)nt * POTRF_FLAGS * sizeof(int32_t)));
+    TTAD_HIP(hipMemset(M->locks.p, 0, (size_t)nt * nt * 16 * sizeof(int32_t)));
     TTAD_HIP(hipMemset(M->tiles.p, 0, off * M->elem));
     DevMat d{};
     d.nt = nt;
@@ -344,6 +346,7 @@ int ttad_matrix_create(ttad_context* ctx, int num_blocks, const int* block_sizes
     d.dinv = M->tab_dinv.as<double*>();
     d.info = M->info.as<int32_t>();
     d.pflags = M->pflags.as<int32_t>();
+    d.locks = M->locks.as<int32_t>();
     M->host_desc = d;
     TTAD_HIP(hipMemcpy(M->desc.p, &d, sizeof(DevMat), hipMemcpyHostToDevice));
     *out = M.release();
@@ -447,7 +450,8 @@ int ttad_matrix_tile(const ttad_matrix* M, int i, int j, void** ptr, int* ld) {
 // ---------------------------------------------------------------------------
 int ttad_potrf_tiled(ttad_context* ctx, ttad_matrix* M, int schedule, int prio_kind, int prec, void* stream,
                      int* info) {
-    if (!ctx || !M || prio_kind < 0 || prio_kind > 4) return set_error(TTAD_ERR_INVALID, "potrf_tiled: bad args");
+    if (!ctx || !M || (prio_kind & ~TTAD_ACCUMULATE) < 0 || (prio_kind & ~TTAD_ACCUMULATE) > 4)
+        return set_error(TTAD_ERR_INVALID, "potrf_tiled: bad args");
     if ((prec == TTAD_F64) != (M->storage == TTAD_F64) || prec < 0 || prec > 3)
         return set_error(TTAD_ERR_INVALID, "potrf_tiled: precision %d does not match the tile storage", prec);
     if (info) *info = 0;
@@ -589,11 +593,14 @@ int ttad_graph_create(ttad_context* ctx, int n_tasks, const ttad_task_desc* task
         if (g->out_ptr[t + 1] < g->out_ptr[t]) return set_error(TTAD_ERR_INVALID, "graph_create: out_ptr not sorted");
         if (indegree[t] < 1) return set_error(TTAD_ERR_GRAPH, "graph_create: task %d has indegree < 1", t);
         const ttad_task_desc& d = tasks[t];
-        if (((d.kind < TTAD_POTRF || d.kind > TTAD_GEMM) && d.kind != TTAD_SPIN) || d.prec < 0 || d.prec > 3 ||
+        const bool kind_ok = (d.kind >= TTAD_POTRF && d.kind <= TTAD_GEMM) || d.kind == TTAD_SPIN ||
+                             d.kind == TTAD_GEMM_ACC || d.kind == TTAD_SYRK_ACC;
+        if (!kind_ok || d.prec < 0 || d.prec > 3 ||
             d.mat < 0 || d.mat > 65535 || (d.kind == TTAD_SPIN && (d.k < 0 || d.k > 10000000)))
             return set_error(TTAD_ERR_INVALID, "graph_create: bad descriptor for task %d", t);
         DevTask& x = g->tasks[t];
-        x.kind = d.kind;
+        x.kind = d.kind == TTAD_GEMM_ACC ? K_GEMM : d.kind == TTAD_SYRK_ACC ? K_SYRK : d.kind;
+        x.accum = d.kind == TTAD_GEMM_ACC || d.kind == TTAD_SYRK_ACC;
         x.prec = d.prec;
         x.mat = d.mat;
         x.i = d.i;
@@ -698,7 +705,7 @@ static int graph_prepare(ttad_graph* g) {
         }
         int needed = 0;
         for (auto& x : g->tasks)
-            if (x.kind != K_NOP) needed |= item_variant(x.kind, x.prec, g->mats[x.mat]->storage);
+            if (x.kind != K_NOP) needed |= item_variant(x.kind, x.prec, g->mats[x.mat]->storage) | (x.accum ? V_ACC : 0);
         g->vset = vset_index(needed);
         g->bucket_items.assign(g->n_buckets, 0);
         g->total_items = 0;
@@ -831,6 +838,7 @@ int ttad_graph_launch(ttad_graph* g, int n_workers, int trace, void* stream) {
             TTAD_HIP(hipMemsetAsync(M->info.p, 0, sizeof(int), g->stream));
             // row-block POTRF flags: cleared by every completed task, and here after a stopped run
             TTAD_HIP(hipMemsetAsync(M->pflags.p, 0, (size_t)M->nt * POTRF_FLAGS * sizeof(int32_t), g->stream));
+            TTAD_HIP(hipMemsetAsync(M->locks.p, 0, (size_t)M->nt * M->nt * 16 * sizeof(int32_t), g->stream));
         }
     TTAD_HIP(hipEventRecord(g->ev_start, g->stream));
     launch_persistent(g->vset, d, g->workers, g->stream);

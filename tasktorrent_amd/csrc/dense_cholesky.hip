@@ -68,8 +68,13 @@ struct BottomLevels {
 
 // Builds the DAG through the Taskflow API exactly as the reference miniapp declares it.
 // ctx == nullptr: dry compile into `dry` (no device).
+// prio_kind may carry TTAD_ACCUMULATE: the accumulate-mode DAG (2d_cholesky.cpp:149-155, 479-608), with
+// the ACCU task fused into its GEMM (the product is added into the tile under a sub-tile lock), so a GEMM
+// releases TRSM(i,j) / POTRF(j) directly, which wait for 1 + j / j contributions.
 static int build_cholesky_graph(ttad_context* ctx, int nb, int storage, int prec, int prio_kind, ttad_graph** out,
                                 Threadpool** dry = nullptr) {
+    const bool acc = (prio_kind & TTAD_ACCUMULATE) != 0;
+    prio_kind &= ~TTAD_ACCUMULATE;
     std::unique_ptr<BottomLevels> bl;
     if (prio_kind == 4) bl = std::make_unique<BottomLevels>(nb);
     auto tp_owned = std::make_unique<Threadpool>(ctx);
@@ -83,7 +88,7 @@ static int build_cholesky_graph(ttad_context* ctx, int nb, int storage, int prec
         .set_fulfill([&](int j) {  // 2d_cholesky.cpp:323-334
             for (int i = j + 1; i < nb; i++) trsm.fulfill_promise({i, j});
         })
-        .set_indegree([](int) { return 1; })
+        .set_indegree([&](int j) { return acc && j > 0 ? j : 1; })  // 2d_cholesky.cpp:345-351
         .set_priority([&](int j) { return bl ? bl->potrf(j) : prio_potrf(prio_kind, nb, j); })
         .set_mapping([&](int j) { return block_2_thread(j, j); })
         .set_name([](int j) { return "POTRF_" + std::to_string(j); });
@@ -94,21 +99,24 @@ static int build_cholesky_graph(ttad_context* ctx, int nb, int storage, int prec
             for (int k = j + 1; k < i; k++) gemm.fulfill_promise({j, i, k});
             for (int k = i; k < nb; k++) gemm.fulfill_promise({j, k, i});
         })
-        .set_indegree([](int2 ij) { return 1 + (ij[1] == 0 ? 0 : 1); })
+        .set_indegree([&](int2 ij) { return acc ? 1 + ij[1] : 1 + (ij[1] == 0 ? 0 : 1); })  // :449-455
         .set_priority([&](int2 ij) { return bl ? bl->trsm(ij[0], ij[1]) : prio_trsm(prio_kind, nb, ij[0], ij[1]); })
         .set_mapping([&](int2 ij) { return block_2_thread(ij[0], ij[1]); })
         .set_name([](int2 ij) { return "TRSM_" + std::to_string(ij[0]) + "_" + std::to_string(ij[1]); });
 
     gemm.set_task([&](int3 kij) {
-            return DeviceTask{kij[1] == kij[2] ? TTAD_SYRK : TTAD_GEMM, prec, 0, kij[1], kij[2], kij[0]};
+            const int kind = kij[1] == kij[2] ? (acc ? TTAD_SYRK_ACC : TTAD_SYRK) : (acc ? TTAD_GEMM_ACC : TTAD_GEMM);
+            return DeviceTask{kind, prec, 0, kij[1], kij[2], kij[0]};
         })
-        .set_fulfill([&](int3 kij) {  // 2d_cholesky.cpp:502-531, non-accumulate branch
+        .set_fulfill([&](int3 kij) {  // 2d_cholesky.cpp:502-531 (accumulate: through ACCU, :575-595)
             const int k = kij[0], i = kij[1], j = kij[2];
-            if (k < j - 1) gemm.fulfill_promise({k + 1, i, j});
+            if (!acc && k < j - 1) gemm.fulfill_promise({k + 1, i, j});
             else if (i == j) potrf.fulfill_promise(i);
             else trsm.fulfill_promise({i, j});
         })
-        .set_indegree([](int3 kij) { return (kij[1] == kij[2] ? 1 : 2) + (kij[0] == 0 ? 0 : 1); })
+        .set_indegree([&](int3 kij) {  // :532-540
+            return (kij[1] == kij[2] ? 1 : 2) + (acc || kij[0] == 0 ? 0 : 1);
+        })
         .set_priority([&](int3 kij) {
             return bl ? bl->gemm(kij[0], kij[1], kij[2]) : prio_gemm(prio_kind, nb, kij[0], kij[1], kij[2]);
         })
@@ -167,7 +175,8 @@ int cholesky_dag_compile(int nb, int prio_kind, std::vector<ttad_task_desc>& des
 extern "C" int ttad_cholesky_dag(int num_blocks, int prio_kind, int* n_tasks, int* n_edges, ttad_task_desc* tasks,
                                  int32_t* indegree, double* priority, int32_t* out_ptr, int32_t* out_idx) {
     using namespace ttad;
-    if (num_blocks <= 0 || num_blocks > 4096 || prio_kind < 0 || prio_kind > 4 || !n_tasks || !n_edges)
+    if (num_blocks <= 0 || num_blocks > 4096 || (prio_kind & ~TTAD_ACCUMULATE) < 0 ||
+        (prio_kind & ~TTAD_ACCUMULATE) > 4 || !n_tasks || !n_edges)
         return set_error(TTAD_ERR_INVALID, "cholesky_dag: bad args");
     Threadpool* tp = nullptr;
     int rc = build_cholesky_graph(nullptr, num_blocks, TTAD_F64, TTAD_F64, prio_kind, nullptr, &tp);

@@ -78,11 +78,12 @@ enum ItemVariant : int {
     V_G32 = 4,       // GEMM/SYRK FP32
     V_GBF16 = 8,     // GEMM/SYRK BF16
     V_GBF16X3 = 16,  // GEMM/SYRK BF16x3
-    V_ALL = 31
+    V_ALL = 31,
+    V_ACC = 32       // accumulate-mode GEMM/SYRK (only in the last set, so the others keep their registers)
 };
 // The compiled sets, smallest first.
-constexpr int N_VSETS = 5;
-constexpr int VSETS[N_VSETS] = {V_F64S, V_F32S | V_G32, V_F32S | V_GBF16, V_F32S | V_GBF16X3, V_ALL};
+constexpr int N_VSETS = 6;
+constexpr int VSETS[N_VSETS] = {V_F64S, V_F32S | V_G32, V_F32S | V_GBF16, V_F32S | V_GBF16X3, V_ALL, V_ALL | V_ACC};
 
 __host__ __device__ inline int item_variant(int kind, int prec, int storage) {
     if (kind == K_GEMM || kind == K_SYRK)
@@ -129,6 +130,13 @@ __device__ __attribute__((noinline)) void potrf_rows_call(TS* A, long lda, int n
     __builtin_amdgcn_s_setprio(3);
     item_potrf_rows<TS>(A, lda, n, dinv, info, col_off, rb, prog, err, spin_limit, lds);
     __builtin_amdgcn_s_setprio(0);
+}
+// Accumulate-mode GEMM items are called out of line too, so the default GEMM loop keeps its registers.
+template <typename TS, int PREC, int VM>
+__device__ __attribute__((noinline)) void gemm_acc_call(const TS* A, long lda, const TS* B, long ldb, TS* C, long ldc,
+                                                        int m, int n, int K, int r0, int c0, bool syrk, int32_t* lock,
+                                                        int32_t* err, int64_t spin_limit, char* lds) {
+    item_gemm_acc<TS, PREC>(A, lda, B, ldb, C, ldc, m, n, K, r0, c0, syrk, -1.0, lock, err, spin_limit, lds);
 }
 template <typename TS, typename TC, int VM>
 __device__ __attribute__((noinline)) void trsm_call(const TS* L, long ldl, const double* dinv, TS* X, long ldx, int m,
@@ -213,32 +221,35 @@ __device__ __forceinline__ bool exec_item(const DevTask& t, int it, const DevMat
             int rb, cb;
             decode_gemm_item(it, (m + GEMM_BM - 1) / GEMM_BM, syrk, rb, cb);
             const int r0 = rb * GEMM_BM, c0 = cb * GEMM_BM;
+            // accumulate mode: one lock word per 128x128 sub-tile (16 per tile, shared beyond 4x4 sub-tiles)
+            int32_t* lock = nullptr;
+            if constexpr ((VM & V_ACC) != 0) {
+                if (t.accum) lock = M.locks + ((long)(i + (long)j * nt) * 16 + (it & 15));
+            } else if (t.accum) {
+                return false;
+            }
+#define TTAD_GEMM_ITEM(TS_, PREC_, CPRE_)                                                                         \
+    if constexpr ((VM & V_ACC) != 0) {                                                                             \
+        if (lock) {                                                                                                \
+            gemm_acc_call<TS_, PREC_, VM>((const TS_*)tile(i, k), m, (const TS_*)tile(j, k), n, (TS_*)tile(i, j), m, \
+                                          m, n, K, r0, c0, syrk, lock, err, spin_limit, lds);                      \
+            return true;                                                                                           \
+        }                                                                                                          \
+    }                                                                                                              \
+    item_gemm<TS_, PREC_, CPRE_>((const TS_*)tile(i, k), m, (const TS_*)tile(j, k), n, (TS_*)tile(i, j), m, m, n,  \
+                                 K, r0, c0, syrk, -1.0, 1.0, lds);                                                 \
+    return true;
             if constexpr ((VM & V_F64S) != 0)
-                if (t.prec == P_F64) {
-                    item_gemm<double, P_F64, true>((const double*)tile(i, k), m, (const double*)tile(j, k), n,
-                                             (double*)tile(i, j), m, m, n, K, r0, c0, syrk, -1.0, 1.0, lds);
-                    return true;
-                }
+                if (t.prec == P_F64) { TTAD_GEMM_ITEM(double, P_F64, true) }
             if constexpr ((VM & V_G32) != 0)
-                if (t.prec == P_F32) {
-                    item_gemm<float, P_F32, true>((const float*)tile(i, k), m, (const float*)tile(j, k), n,
-                                            (float*)tile(i, j), m, m, n, K, r0, c0, syrk, -1.0, 1.0, lds);
-                    return true;
-                }
+                if (t.prec == P_F32) { TTAD_GEMM_ITEM(float, P_F32, true) }
             // BF16 products read C after the last K-chunk (measured 11% faster per bf16x3 item than
             // reading it up front, which pays off for the FP64/FP32 pipelines)
             if constexpr ((VM & V_GBF16) != 0)
-                if (t.prec == P_BF16) {
-                    item_gemm<float, P_BF16, false>((const float*)tile(i, k), m, (const float*)tile(j, k), n,
-                                             (float*)tile(i, j), m, m, n, K, r0, c0, syrk, -1.0, 1.0, lds);
-                    return true;
-                }
+                if (t.prec == P_BF16) { TTAD_GEMM_ITEM(float, P_BF16, false) }
             if constexpr ((VM & V_GBF16X3) != 0)
-                if (t.prec == P_BF16X3) {
-                    item_gemm<float, P_BF16X3, false>((const float*)tile(i, k), m, (const float*)tile(j, k), n,
-                                               (float*)tile(i, j), m, m, n, K, r0, c0, syrk, -1.0, 1.0, lds);
-                    return true;
-                }
+                if (t.prec == P_BF16X3) { TTAD_GEMM_ITEM(float, P_BF16X3, false) }
+#undef TTAD_GEMM_ITEM
             return false;
         }
         default:

@@ -378,6 +378,7 @@ hipError_t kernels_init() {
     TTAD_SETLDS2(k_persistent<VSETS[2]>, vset_lds(VSETS[2]));
     TTAD_SETLDS2(k_persistent<VSETS[3]>, vset_lds(VSETS[3]));
     TTAD_SETLDS2(k_persistent<VSETS[4]>, vset_lds(VSETS[4]));
+    TTAD_SETLDS2(k_persistent<VSETS[5]>, vset_lds(VSETS[5]));
     TTAD_SETLDS2((k_raw_gemm<double, P_F64>), (gemm_item_lds<double, P_F64>()));
     TTAD_SETLDS2((k_raw_gemm<float, P_F32>), (gemm_item_lds<float, P_F32>()));
     TTAD_SETLDS2((k_raw_gemm<float, P_BF16>), (gemm_item_lds<float, P_BF16>()));
@@ -396,7 +397,8 @@ static const void* persistent_fn(int vset) {
         case 1: return (const void*)k_persistent<VSETS[1]>;
         case 2: return (const void*)k_persistent<VSETS[2]>;
         case 3: return (const void*)k_persistent<VSETS[3]>;
-        default: return (const void*)k_persistent<VSETS[4]>;
+        case 4: return (const void*)k_persistent<VSETS[4]>;
+        default: return (const void*)k_persistent<VSETS[5]>;
     }
 }
 
@@ -406,7 +408,8 @@ static int vset_lds_of(int vset) {
         case 1: return vset_lds(VSETS[1]);
         case 2: return vset_lds(VSETS[2]);
         case 3: return vset_lds(VSETS[3]);
-        default: return vset_lds(VSETS[4]);
+        case 4: return vset_lds(VSETS[4]);
+        default: return vset_lds(VSETS[5]);
     }
 }
 
@@ -422,7 +425,8 @@ void launch_items(int vset, const DevMat* mats, const DevTask* tasks, const int3
                   int n_items, int32_t* err, hipStream_t s) {
     if (n_items <= 0) return;
     const dim3 grid(n_items), block(NTHREADS);
-    const int lds = vset_lds_of(vset);
+    // (no accumulate-mode variant here: host-sequenced plans hold no accumulate tasks)
+    const int lds = vset_lds_of(std::min(vset, N_VSETS - 2));
     switch (vset) {
         case 0: hipLaunchKernelGGL(k_items<VSETS[0]>, grid, block, lds, s, mats, tasks, prefix, n_tasks, err); break;
         case 1: hipLaunchKernelGGL(k_items<VSETS[1]>, grid, block, lds, s, mats, tasks, prefix, n_tasks, err); break;
@@ -445,7 +449,8 @@ void launch_persistent(int vset, const DevGraph& g, int n_workers, hipStream_t s
         case 1: hipLaunchKernelGGL(k_persistent<VSETS[1]>, grid, block, lds, s, g); break;
         case 2: hipLaunchKernelGGL(k_persistent<VSETS[2]>, grid, block, lds, s, g); break;
         case 3: hipLaunchKernelGGL(k_persistent<VSETS[3]>, grid, block, lds, s, g); break;
-        default: hipLaunchKernelGGL(k_persistent<VSETS[4]>, grid, block, lds, s, g); break;
+        case 4: hipLaunchKernelGGL(k_persistent<VSETS[4]>, grid, block, lds, s, g); break;
+        default: hipLaunchKernelGGL(k_persistent<VSETS[5]>, grid, block, lds, s, g); break;
     }
 }
 

@@ -82,7 +82,7 @@ struct ttad_matrix {
     int nt = 0, n = 0, storage = TTAD_F64;
     std::vector<int> bsz, boff;
     std::vector<size_t> tile_off;  // element offset of tile (i,j) (i >= j), index i + j*nt
-    ttad::DevBuf tiles, dinv, tab_bsz, tab_boff, tab_tiles, tab_dinv, info, desc, pflags;
+    ttad::DevBuf tiles, dinv, tab_bsz, tab_boff, tab_tiles, tab_dinv, info, desc, pflags, locks;
     size_t elem = 8;
     long long bytes = 0;
     ttad::DevMat host_desc{};

@@ -488,6 +488,62 @@ __device__ __forceinline__ void item_gemm(const TS* __restrict__ A, long lda, co
     }
 }
 
+// Accumulate-mode GEMM / SYRK item (2d_cholesky.cpp:479-501 with beta = 0, then the ACCU task
+// `*blocks[i+j*nb] += *Atmp`, :570, fused): the product is formed from zero and added into C while
+// holding `lock`, the sub-tile's lock word (agent-scope acquire / release), so contributions of
+// different k land in any order but one at a time. A lock wait gives up after spin_limit ticks or
+// once *err is nonzero (and sets *err = 1 when it times out).
+template <typename TS, int PREC>
+__device__ __forceinline__ void item_gemm_acc(const TS* __restrict__ A, long lda, const TS* __restrict__ B, long ldb,
+                                              TS* __restrict__ C, long ldc, int m, int n, int K, int r0, int c0,
+                                              bool syrk, double alpha, int32_t* lock, int32_t* err,
+                                              int64_t spin_limit, char* lds) {
+    const int mv = min(GEMM_BM, m - r0), nv = min(GEMM_BM, n - c0);
+    if (mv <= 0 || nv <= 0) return;
+    auto body = [&](auto* gsel) {
+        using G = typename std::remove_pointer<decltype(gsel)>::type;
+        typedef typename G::acc_t acc_t;
+        acc_t acc[G::WM][G::WN];
+        G::zero(acc);
+        G::run(acc, A + r0, lda, mv, B + c0, ldb, nv, K, lds);
+        if (threadIdx.x == 0) {
+            const int64_t t0 = (int64_t)wall_clock64();
+            int expected = 0;
+            while (!__hip_atomic_compare_exchange_strong(lock, &expected, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT)) {
+                expected = 0;
+                if ((int64_t)wall_clock64() - t0 > spin_limit ||
+                    __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the previous holder's additions
+        }
+        __syncthreads();
+#pragma unroll
+        for (int a = 0; a < G::WM; a++)
+#pragma unroll
+            for (int b = 0; b < G::WN; b++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int rr = G::row_of(a, r), cc = G::col_of(b, r);
+                    if (rr < mv && cc < nv && (!syrk || r0 + rr >= c0 + cc)) {
+                        TTAD_GLOBAL TS* p = gmem(C) + (r0 + rr) + (long)(c0 + cc) * ldc;
+                        *p = (TS)((double)(*p) + alpha * (double)acc[a][b][r]);
+                    }
+                }
+        __syncthreads();
+        if (threadIdx.x == 0) __hip_atomic_store(lock, 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    if constexpr (PREC == P_BF16 || PREC == P_BF16X3) {
+        body((GemmNTbf16<TS, GEMM_BM, GEMM_BM, PREC == P_BF16X3, true>*)nullptr);
+    } else {
+        body((GemmNT<TS, TS, TS, GEMM_BM, GEMM_BM, gemm_bk<TS>(), true, true>*)nullptr);
+    }
+}
+
 // Decode the (row block, col block) of GEMM/SYRK item `it` in a tile of tr x tc sub-tiles.
 __device__ __forceinline__ void decode_gemm_item(int it, int tr, bool syrk, int& rb, int& cb) {
     if (!syrk) {

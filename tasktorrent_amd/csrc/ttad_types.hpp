@@ -26,9 +26,10 @@ struct DevMat {
     double* const* dinv;   // [nt]: inverses of the 64x64 diagonal blocks of L_jj (FP64, 64x64 each, ld 64)
     int32_t* info;         // first non-positive pivot, 1-based global column; 0 = none
     int32_t* pflags;       // [nt * 128]: progress flags of the row-block POTRF items of each diagonal tile (zero between runs)
+    int32_t* locks;        // [nt * nt * 16]: accumulate-mode sub-tile locks of tile (i, j) (zero when free)
 };
 
-// 32-byte device task record.
+// Device task record.
 struct DevTask {
     int32_t kind;
     int32_t prec;
@@ -37,6 +38,7 @@ struct DevTask {
     int32_t i, j, k;
     int32_t bucket;
     int32_t crit;  // on a longest path of the DAG: runs with no other item started on its CU
+    int32_t accum; // GEMM/SYRK: add the product into C under the sub-tile lock (accumulate mode)
 };
 
 // Work items per task; host and device must agree.

@@ -101,3 +101,26 @@ def test_dag_bottom_level_priorities(nb):
     names = [tt.task_name(t) for t in tasks]
     assert prio[names.index(f"POTRF_{nb - 1}")] == 2.0
     assert prio[names.index("POTRF_0")] == prio.max()
+
+
+@pytest.mark.parametrize("nb", [1, 2, 5, 9])
+def test_dag_accumulate_mode(nb):
+    # 2d_cholesky.cpp --accumulate (:345-351, 449-455, 532-540), with the ACCU task fused into its GEMM:
+    # no GEMM -> GEMM edges; POTRF(j) waits for the j updates of (j, j), TRSM(i, j) for POTRF(j) and j updates.
+    tasks, indeg, prio, out_ptr, out_idx = tt.cholesky_dag(nb, 2 | tt.TTAD_ACCUMULATE)
+    kinds = [t[0] for t in tasks]
+    assert kinds.count(tt.TTAD_POTRF) == nb
+    assert kinds.count(tt.TTAD_TRSM) == nb * (nb - 1) // 2
+    assert kinds.count(tt.TTAD_SYRK_ACC) == nb * (nb - 1) // 2
+    assert kinds.count(tt.TTAD_GEMM_ACC) == sum((nb - k - 1) * (nb - k - 2) // 2 for k in range(nb))
+    for t, d in zip(tasks, indeg):
+        kind, _, _, i, j, k = t
+        expect = {tt.TTAD_POTRF: max(1, j), tt.TTAD_TRSM: 1 + j, tt.TTAD_SYRK_ACC: 1, tt.TTAD_GEMM_ACC: 2}[kind]
+        assert d == expect, (t, d)
+    incoming = np.bincount(out_idx, minlength=len(tasks))
+    seed = np.array([1 if tt.task_name(t) == "POTRF_0" else 0 for t in tasks])
+    assert np.array_equal(incoming + seed, indeg)
+    upd = {tt.TTAD_GEMM_ACC, tt.TTAD_SYRK_ACC}
+    for u in range(len(tasks)):
+        if kinds[u] in upd:
+            assert not any(kinds[v] in upd for v in out_idx[out_ptr[u]:out_ptr[u + 1]])

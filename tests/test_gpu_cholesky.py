@@ -154,3 +154,18 @@ def test_spin_bench_scheduler_overhead(ctx, nb, workers):
     assert r["elapsed_s"] <= 2.0 * lower + 5e-3
     assert 0.0 < r["efficiency"] <= 1.0
     assert r["efficiency"] == pytest.approx(len(tasks) * spin_us * 1e-6 / (r["elapsed_s"] * r["workers"]))
+
+
+@pytest.mark.parametrize("prec", ["f64", "bf16x3"])
+def test_accumulate_mode_matches_oracle(ctx, oracle, prec):
+    # accumulate mode adds the updates of a tile in the order they finish: not bitwise
+    # reproducible, but the same factor to rounding
+    n, b = 1536, 256
+    A = oracle.spd_bbt(n, 17)
+    Lref = np.linalg.cholesky(A)
+    L, info = factor(ctx, A, [b] * 6, tt.TTAD_SCHED_PERSISTENT, prio=2 | tt.TTAD_ACCUMULATE, prec=prec)
+    assert info == 0
+    tol = {"f64": 1e-12, "bf16x3": 1e-4}[prec]
+    assert rel(L, Lref) < tol
+    L2, _ = factor(ctx, A, [b] * 6, tt.TTAD_SCHED_PERSISTENT, prec=prec)
+    assert rel(L, L2) < tol
