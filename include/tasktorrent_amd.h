@@ -178,6 +178,9 @@ int ttad_graph_wait(ttad_graph* g, double timeout_s, ttad_graph_stats* stats);
 /* Writes the last run's trace in the reference Logger CSV format
  * (util.cpp:117-127): "<prefix><worker>>run>NAME,start,end" in seconds. */
 int ttad_graph_write_trace(ttad_graph* g, const char* path, const char* prefix);
+/* Writes the graph's edges in the reference DepsLogger CSV format (util.cpp:103-112):
+ * one "SRC_NAME,DST_NAME" line per fulfill_promise edge. No run needed. */
+int ttad_graph_write_deps(ttad_graph* g, const char* path);
 int ttad_graph_destroy(ttad_graph* g);
 
 /* ---- solve phase ------------------------------------------------------------- */

@@ -264,6 +264,10 @@ class Graph:
     def write_trace(self, path: str, prefix: str = "Wk_"):
         _capi.check(lib().ttad_graph_write_trace(self._h, path.encode(), prefix.encode()))
 
+    def write_deps(self, path: str):
+        """Edges as the reference DepsLogger writes them: one "SRC,DST" line per edge."""
+        _capi.check(lib().ttad_graph_write_deps(self._h, path.encode()))
+
     def close(self):
         if self._h:
             lib().ttad_graph_destroy(self._h)

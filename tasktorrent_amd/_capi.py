@@ -114,6 +114,7 @@ _SIGS = {
     "ttad_graph_launch": (I, [V, I, I, V]),
     "ttad_graph_wait": (I, [V, D, ctypes.POINTER(GraphStats)]),
     "ttad_graph_write_trace": (I, [V, ctypes.c_char_p, ctypes.c_char_p]),
+    "ttad_graph_write_deps": (I, [V, ctypes.c_char_p]),
     "ttad_graph_destroy": (I, [V]),
     "ttad_solve": (I, [V, V, I, V, LL, V, LL, V]),
     "ttad_residual": (I, [V, V, I, V, LL, V, LL, V, LL, ctypes.POINTER(D), V]),

@@ -896,6 +896,7 @@ static std::string task_label(const DevTask& t) {
     char b[96];
     if (t.kind == K_POTRF) snprintf(b, sizeof(b), "POTRF_%d", t.j);
     else if (t.kind == K_TRSM) snprintf(b, sizeof(b), "TRSM_%d_%d", t.i, t.j);
+    else if (t.kind == K_NOP) snprintf(b, sizeof(b), "SPIN_%d_%d", t.i, t.j);
     else snprintf(b, sizeof(b), "GEMM_%d_%d_%d", t.k, t.i, t.j);
     return b;
 }
@@ -924,6 +925,17 @@ int ttad_graph_write_trace(ttad_graph* g, const char* path, const char* prefix) 
           << (double)(rec[4 * r + 2] - t0) * tick << "," << (double)(rec[4 * r + 3] - t0) * tick << "\n";
     }
     return TTAD_OK;
+}
+
+int ttad_graph_write_deps(ttad_graph* g, const char* path) {
+    if (!g || !path) return set_error(TTAD_ERR_INVALID, "graph_write_deps: bad args");
+    std::ofstream f(path);
+    if (!f) return set_error(TTAD_ERR_INVALID, "graph_write_deps: cannot open %s", path);
+    for (int t = 0; t < g->n_tasks; t++) {
+        const std::string src = task_label(g->tasks[t]);
+        for (int e = g->out_ptr[t]; e < g->out_ptr[t + 1]; e++) f << src << "," << task_label(g->tasks[g->out_idx[e]]) << "\n";
+    }
+    return f ? TTAD_OK : set_error(TTAD_ERR_INVALID, "graph_write_deps: write failed");
 }
 
 int ttad_graph_destroy(ttad_graph* g) {

@@ -169,3 +169,20 @@ def test_accumulate_mode_matches_oracle(ctx, oracle, prec):
     assert rel(L, Lref) < tol
     L2, _ = factor(ctx, A, [b] * 6, tt.TTAD_SCHED_PERSISTENT, prec=prec)
     assert rel(L, L2) < tol
+
+
+def test_graph_write_deps_matches_oracle(ctx, oracle, tmp_path):
+    # the device graph's edges in the reference DepsLogger format (util.cpp:103-112), against the
+    # oracle's Taskflow run of the same DAG
+    nb = 5
+    tasks, indeg, prio, out_ptr, out_idx = tt.cholesky_dag(nb, 2)
+    G = tt.Graph(ctx, tasks, indeg, out_ptr, out_idx, prio)
+    ours = tmp_path / "ours.csv"
+    G.write_deps(str(ours))
+    G.close()
+    A = oracle.spd_bbt(nb * 8, 1)
+    ref = tmp_path / "ref.csv"
+    oracle.cholesky(A, 8, n_threads=2, prio_kind=2, deps_path=str(ref))
+    lines = ours.read_text().splitlines()
+    assert len(lines) == len(set(lines)) == len(out_idx)
+    assert set(lines) == {line for line in ref.read_text().splitlines() if line}
