@@ -180,6 +180,38 @@ def serving_leg(ctx, rate: float, n_req: int, seed: int = 7, prec: str = "bf16x3
             "sizes": "n = 64*round(exp(U(ln64, ln4096))/64), 1-8 RHS, ~10% cached-factor requests"}
 
 
+def trailing_update_leg(ctx, mn: int, k: int, reps: int = 10):
+    """FP64 C -= A B^T with m = n = mn (ttad_dgemm, the GEMM items the factorization runs, as one plain grid),
+    timed with events on the launch stream, beside rocBLAS DGEMM (torch.addmm) on the same operands."""
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(11)
+    At = torch.randn((k, mn), dtype=torch.float64, device="cuda", generator=g)  # column-major mn x k
+    Bt = torch.randn((k, mn), dtype=torch.float64, device="cuda", generator=g)
+    C = torch.randn((mn, mn), dtype=torch.float64, device="cuda", generator=g)
+    fl = 2.0 * mn * mn * k
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / reps
+
+    t_ours = timed(lambda: ctx.gemm(mn, mn, k, -1.0, At, mn, Bt, mn, 1.0, C, mn))
+    t_blas = timed(lambda: C.addmm_(Bt.t(), At, alpha=-1.0))
+    del At, Bt, C
+    torch.cuda.empty_cache()
+    return {"shape": f"m=n={mn} k={k}", "ms": round(t_ours * 1e3, 3),
+            "tflops": round(fl / t_ours / 1e12, 3), "frac": round(fl / t_ours / 1e12 / PEAK["f64"], 4),
+            "rocblas_tflops": round(fl / t_blas / 1e12, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,6 +228,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson request rate of the serving leg (1/s)")
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-update", action="store_true", help="skip the FP64 trailing-update kernel leg")
+    ap.add_argument("--update-mn", type=int, default=16384, help="m = n of the trailing-update kernel leg")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device copy first (reference kernel for HBM byte counters)")
     ap.add_argument("--prio", type=int, default=2,
@@ -262,6 +296,11 @@ def main():
                   "factor_tflops": round(F / float(np.median(fac64)) / 1e12, 3),
                   "tts_tflops": round(F / float(np.median(tts64)) / 1e12, 3),
                   "backward_error": st["backward_error"], "ir_steps": st["iterations"]}
+
+    # ---- FP64 trailing-update kernel alone (C -= A B^T, the shape of one panel's update) ----
+    if not args.no_update:
+        res["f64_trailing_update"] = trailing_update_leg(ctx, args.update_mn, b)
+        log(f"trailing update: {res['f64_trailing_update']}")
 
     # ---- mixed legs ----
     def mixed_step(prec):
