@@ -56,6 +56,20 @@ def test_dgemm_beta0_alpha(ctx, oracle):
     assert rel(from_dev(dC), ref) < 1e-13
 
 
+@pytest.mark.parametrize("alpha,beta", [(0.5, -2.0), (-1.0, 0.25), (1.0, 1.0)])
+def test_dgemm_general_alpha_beta(ctx, oracle, alpha, beta):
+    rng = np.random.default_rng(17)
+    m, n, k = 257, 190, 140
+    A = np.asfortranarray(rng.standard_normal((m, k)))
+    B = np.asfortranarray(rng.standard_normal((n, k)))
+    C = np.asfortranarray(rng.standard_normal((m, n)))
+    dA, dB, dC = to_dev(A), to_dev(B), to_dev(C)
+    ctx.gemm(m, n, k, alpha, dA, m, dB, n, beta, dC, m)
+    ref = C.copy(order="F")
+    oracle.dgemm_nt(alpha, A, B, beta, ref)
+    assert rel(from_dev(dC), ref) < 1e-13
+
+
 @pytest.mark.parametrize("n,k", [(128, 64), (512, 512), (300, 77), (1, 1)])
 def test_dsyrk_lower_only(ctx, oracle, n, k):
     rng = np.random.default_rng(n + k)
