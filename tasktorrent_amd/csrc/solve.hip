@@ -75,6 +75,21 @@ __global__ void k_zero_cols(double* X, long ld, int n, int nrhs) {
     for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < tot; x += (long)gridDim.x * blockDim.x)
         X[(x % n) + (x / n) * ld] = 0.0;
 }
+// Largest |B| over a column-major n x nrhs array (bit pattern of a non-negative double, so
+// integer order is numeric order and a NaN ranks above every number), folded into *out.
+__global__ void k_absmax_cols(const double* B, long ldb, int n, int nrhs, unsigned long long* out) {
+    unsigned long long m = 0;
+    const long tot = (long)n * nrhs;
+    for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < tot; x += (long)gridDim.x * blockDim.x) {
+        const unsigned long long v = (unsigned long long)__double_as_longlong(fabs(B[(x % n) + (x / n) * ldb]));
+        m = v > m ? v : m;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_down(m, off);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
+}
 __global__ void k_copy_cols(const double* S, long lds_, double* D, long ldd, int n, int nrhs) {
     const long tot = (long)n * nrhs;
     for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < tot; x += (long)gridDim.x * blockDim.x)
@@ -218,17 +233,36 @@ static int check_err_flag(ttad_context* ctx, hipStream_t s) {
 
 // Refinement loop: X := X + L^{-T}L^{-1}(B - A X) until eta <= tol; stops early when
 // eta no longer halves (stall). Returns TTAD_ERR_IR_STALLED if tol is not reached.
+// X = 0 on entry, so the first residual is B itself and needs no product with A: its
+// backward error is max|b| / (||A|| * 0 + max|b|) = 1 for any nonzero B (0 for B = 0), and
+// the first correction is solved from B directly.
 static int refine(ttad_context* ctx, ttad_matrix* A, ttad_matrix* L, int nrhs, const double* B, long ldb, double* X,
                   long ldx, double tol, int max_iter, int* iters, double* eta_out, hipStream_t s) {
     const int n = A->n;
     int rc = ensure(ctx->sw_R, (size_t)n * nrhs * 8);
     if (rc) return rc;
+    if ((rc = ensure(ctx->sw_norms, (size_t)(3 * 32 + 8) * 8))) return rc;
     double* R = ctx->sw_R.as<double>();
     double eta = INFINITY, prev = INFINITY;
     int it = 0, stall = 0;
-    if ((rc = residual(ctx, A, nrhs, B, ldb, X, ldx, R, n, &eta, s))) return rc;
+    {
+        unsigned long long* norms = ctx->sw_norms.as<unsigned long long>();
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(128), 0, s, norms, 1);
+        hipLaunchKernelGGL(k_absmax_cols, dim3(256), dim3(256), 0, s, B, ldb, n, nrhs, norms);
+        TTAD_HIP(hipGetLastError());
+        unsigned long long h = 0;
+        TTAD_HIP(hipMemcpyAsync(&h, norms, 8, hipMemcpyDeviceToHost, s));
+        TTAD_HIP(hipStreamSynchronize(s));
+        double bmax;
+        memcpy(&bmax, &h, 8);
+        eta = std::isnan(bmax) ? INFINITY : (bmax > 0 ? 1.0 : 0.0);
+    }
     while (eta > tol && it < max_iter) {
-        if ((rc = solve_add(ctx, L, nrhs, R, n, X, ldx, s))) return rc;
+        if (it == 0)
+            rc = solve_add(ctx, L, nrhs, B, ldb, X, ldx, s);
+        else
+            rc = solve_add(ctx, L, nrhs, R, n, X, ldx, s);
+        if (rc) return rc;
         it++;
         prev = eta;
         if ((rc = residual(ctx, A, nrhs, B, ldb, X, ldx, R, n, &eta, s))) return rc;

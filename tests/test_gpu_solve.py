@@ -112,3 +112,31 @@ def test_factor_reused_for_new_rhs(ctx, oracle):
         dB, dX = _dev_cols(B), _dev_cols(np.zeros((n, 2)))
         st = ctx.solve_ir(M, W, 2, dB, dX, tol=1e-13, max_iter=20)
         assert st["rc"] == 0 and backward_error(A, from_dev(dX), B) <= 2e-13
+
+
+def test_solve_ir_zero_rhs_and_padded_ld(ctx, oracle):
+    # the refinement starts at X = 0, where the residual is B itself: a zero B needs no step,
+    # a zero column stays exactly zero, and padding beyond n rows (ld > n) is never read
+    import torch
+    n, b = 768, 256
+    A = oracle.spd_bbt(n, 3)
+    M = _mat(ctx, A, b)
+    W = tt.TiledMatrix(ctx, M.block_sizes, "f64")
+    W.copy_from(M)
+    assert ctx.potrf_tiled(W, tt.TTAD_SCHED_PERSISTENT, 2, "f64") == 0
+
+    dZ, dX0 = _dev_cols(np.zeros((n, 2))), _dev_cols(np.ones((n, 2)))
+    st = ctx.solve_ir(M, W, 2, dZ, dX0, tol=1e-13, max_iter=20)
+    assert st["rc"] == 0 and st["iterations"] == 0 and st["backward_error"] == 0.0
+    assert not from_dev(dX0).any()
+
+    B = np.random.default_rng(4).standard_normal((n, 3))
+    B[:, 0] = 0.0
+    dB = torch.full((3, n + 7), float("nan"), dtype=torch.float64, device="cuda")
+    dB[:, :n] = torch.from_numpy(B.T.copy()).cuda()
+    dX = torch.zeros((3, n + 5), dtype=torch.float64, device="cuda")
+    st = ctx.solve_ir(M, W, 3, dB, dX, tol=1e-13, max_iter=20)
+    X = dX[:, :n].cpu().numpy().T
+    assert st["rc"] == 0 and st["iterations"] >= 1
+    assert not X[:, 0].any()
+    assert backward_error(A, X[:, 1:], B[:, 1:]) <= 2e-13
