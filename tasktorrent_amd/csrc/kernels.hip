@@ -343,15 +343,35 @@ __global__ void k_unpack(double* __restrict__ dst, long ldd, const DevMat* M, in
         dst[(ri + p) + (long)(cj + q) * ldd] = (double)t[e];
     }
 }
-// Tile-wise copy with conversion between two matrices of the same partition.
+// Tile-wise copy with conversion between two matrices of the same partition (one workgroup of
+// 256 per lower tile). Element pairs move as one vector access, four pairs in flight per thread;
+// a tile whose pointers or element count do not allow pairs is copied element by element.
 template <typename TD, typename TSRC>
-__global__ void k_copy_tiles(const DevMat* S, const DevMat* D, int nt) {
+__global__ __launch_bounds__(256) void k_copy_tiles(const DevMat* S, const DevMat* D, int nt) {
+    typedef TSRC s2 __attribute__((ext_vector_type(2)));
+    typedef TD d2 __attribute__((ext_vector_type(2)));
     const int i = blockIdx.x, j = blockIdx.y;
     if (i < j) return;
     const TSRC* s = (const TSRC*)S->tiles[i + (long)j * nt];
     TD* d = (TD*)D->tiles[i + (long)j * nt];
     const long cnt = (long)S->bsz[i] * S->bsz[j];
-    for (long e = threadIdx.x; e < cnt; e += blockDim.x) d[e] = (TD)s[e];
+    const bool pairs = (cnt % 2) == 0 && ((uintptr_t)s % sizeof(s2)) == 0 && ((uintptr_t)d % sizeof(d2)) == 0;
+    if (!pairs) {
+        for (long e = threadIdx.x; e < cnt; e += 256) d[e] = (TD)s[e];
+        return;
+    }
+    const TTAD_GLOBAL s2* sp = gmem((const s2*)s);
+    TTAD_GLOBAL d2* dp = gmem((d2*)d);
+    const long np = cnt / 2;
+    long e = threadIdx.x;
+    for (; e + 3 * 256 < np; e += 4 * 256) {
+        const s2 v0 = sp[e], v1 = sp[e + 256], v2 = sp[e + 512], v3 = sp[e + 768];
+        dp[e] = __builtin_convertvector(v0, d2);
+        dp[e + 256] = __builtin_convertvector(v1, d2);
+        dp[e + 512] = __builtin_convertvector(v2, d2);
+        dp[e + 768] = __builtin_convertvector(v3, d2);
+    }
+    for (; e < np; e += 256) dp[e] = __builtin_convertvector(sp[e], d2);
 }
 
 // ---------------------------------------------------------------------------
