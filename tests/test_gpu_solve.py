@@ -140,3 +140,21 @@ def test_solve_ir_zero_rhs_and_padded_ld(ctx, oracle):
     assert st["rc"] == 0 and st["iterations"] >= 1
     assert not X[:, 0].any()
     assert backward_error(A, X[:, 1:], B[:, 1:]) <= 2e-13
+
+
+@pytest.mark.parametrize("dst", ["f32", "f64"])
+def test_matrix_copy_ragged_tiles(ctx, dst):
+    # tiles of 130, 67 and 3 rows: element counts both even (copied in pairs) and odd (one by one);
+    # FP64 -> FP32 rounds to nearest like numpy's astype
+    rng = np.random.default_rng(3)
+    sizes = [130, 67, 3]
+    n = sum(sizes)
+    A = rng.standard_normal((n, n))
+    M = tt.TiledMatrix(ctx, sizes, "f64")
+    M.set(A)
+    W = tt.TiledMatrix(ctx, sizes, dst)
+    W.copy_from(M)
+    exp = np.tril(A)
+    if dst == "f32":
+        exp = exp.astype(np.float32).astype(np.float64)
+    assert np.array_equal(W.get(), exp)
