@@ -212,6 +212,53 @@ def trailing_update_leg(ctx, mn: int, k: int, reps: int = 10):
             "rocblas_tflops": round(fl / t_blas / 1e12, 3)}
 
 
+def panel_leg(ctx, m: int, b: int, reps: int = 10):
+    """The panel steps alone, FP64: ttad_dpotrf on one b x b tile (synchronous: the event bracket includes its
+    info readback) and ttad_dtrsm on an m x b column of tiles (B := B L^-T, the first panel's TRSMs at m = n - b),
+    with the TRSM's HBM rate counted as one read and one write of the m x b panel."""
+    import numpy as np
+    import torch
+    m = max(m, b)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(13)
+    Bm = torch.randn((b, b), dtype=torch.float64, device="cuda", generator=g)
+    A = Bm @ Bm.T / b + torch.eye(b, dtype=torch.float64, device="cuda")  # symmetric: row- = column-major
+    Ls = torch.empty_like(A)
+    t_potrf = []
+    for r in range(reps + 2):
+        Ls.copy_(A)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        info = ctx.potrf(Ls, b, b)
+        e1.record()
+        torch.cuda.synchronize()
+        if info != 0:
+            raise RuntimeError(f"panel leg: potrf info={info}")
+        if r >= 2:
+            t_potrf.append(e0.elapsed_time(e1) / 1e3)
+    P = torch.randn((b, m), dtype=torch.float64, device="cuda", generator=g)  # column-major m x b
+    for _ in range(2):
+        ctx.trsm(m, b, Ls, b, P, m)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        ctx.trsm(m, b, Ls, b, P, m)
+    e1.record()
+    torch.cuda.synchronize()
+    t_trsm = e0.elapsed_time(e1) / 1e3 / reps
+    if not bool(torch.isfinite(P).all()):
+        raise RuntimeError("panel leg: TRSM produced non-finite values")
+    tp = float(np.median(t_potrf))
+    gbps = 2.0 * m * b * 8 / t_trsm / 1e9
+    del A, Ls, P, Bm
+    torch.cuda.empty_cache()
+    return {"potrf_tile": b, "potrf_ms": round(tp * 1e3, 4), "potrf_tflops": round(b ** 3 / 3 / tp / 1e12, 4),
+            "trsm_shape": f"m={m} n={b}", "trsm_ms": round(t_trsm * 1e3, 4), "trsm_gbps": round(gbps, 1),
+            "trsm_hbm_frac": round(gbps / 8000.0, 4), "trsm_tflops": round(float(m) * b * b / t_trsm / 1e12, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,7 +275,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson request rate of the serving leg (1/s)")
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--no-update", action="store_true", help="skip the FP64 trailing-update kernel leg")
+    ap.add_argument("--no-update", action="store_true", help="skip the FP64 trailing-update and panel-step legs")
     ap.add_argument("--update-mn", type=int, default=16384, help="m = n of the trailing-update kernel leg")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device copy first (reference kernel for HBM byte counters)")
@@ -301,6 +348,8 @@ def main():
     if not args.no_update:
         res["f64_trailing_update"] = trailing_update_leg(ctx, args.update_mn, b)
         log(f"trailing update: {res['f64_trailing_update']}")
+        res["f64_panel"] = panel_leg(ctx, n - b, b)
+        log(f"panel steps: {res['f64_panel']}")
 
     # ---- mixed legs ----
     def mixed_step(prec):
