@@ -280,7 +280,7 @@ def main():
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device copy first (reference kernel for HBM byte counters)")
     ap.add_argument("--prio", type=int, default=2,
-                    help="PrioKind of the FP64 leg and the CPU baseline (the mixed legs use 2 as well: critical path first)")
+                    help="PrioKind of the FP64 leg and the CPU baseline (the mixed legs use TTAD_MIXED_PRIO, default 4: bottom level)")
     args = ap.parse_args()
 
     if args.gpus != 1:

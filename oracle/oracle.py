@@ -13,7 +13,7 @@ What it restates (reference file:line):
   * solve_check()     2d_cholesky.cpp:646-689 (two triangular solves, forward error)
   * the tile kernels  LAPACKE_dpotrf / cblas_dtrsm / cblas_dsyrk / cblas_dgemm as
                       called at 2d_cholesky.cpp:319, 404-405, 490-494
-The metrics (residual, backward error) are the north-star report quantities.
+The metrics (factor residual, backward error) are the quantities the solver reports.
 """
 from __future__ import annotations
 
