@@ -563,14 +563,18 @@ static void mark_critical(ttad_graph& g) {
     for (int t = 0; t < n; t++) g.tasks[t].crit = top[t] + bot[t] - w[t] == len;
 }
 
-// GEMM/SYRK work queues homed per XCD (default); TTAD_XCD_AFFINITY=0 turns them into one
-// shared queue per priority bucket (for comparisons).
-static bool xcd_affinity() {
-    static const bool on = [] {
+// Which GEMM/SYRK work queues are homed per XCD. By default the FP32 / BF16 / bf16x3 updates are
+// (measured at n = 32768: bf16x3 time to solution 128.9 ms homed, 144.2 ms shared) and the FP64 ones share
+// one queue per priority bucket (FP64 factorization 223.7 ms shared, 229.4 ms homed); spin tasks are
+// homed. TTAD_XCD_AFFINITY=0 shares every queue, TTAD_XCD_AFFINITY=1 homes FP64 updates too.
+static bool xcd_homed(int kind, int prec) {
+    static const int mode = [] {
         const char* v = getenv("TTAD_XCD_AFFINITY");
-        return !(v && v[0] == '0');
+        return v ? (v[0] == '0' ? 0 : 1) : -1;
     }();
-    return on;
+    if (kind == K_NOP) return mode != 0;
+    if (kind != K_GEMM && kind != K_SYRK) return false;
+    return mode == 1 || (mode < 0 && prec != P_F64);
 }
 
 int ttad_graph_create(ttad_context* ctx, int n_tasks, const ttad_task_desc* tasks, const int32_t* indegree,
@@ -633,7 +637,7 @@ int ttad_graph_create(ttad_context* ctx, int n_tasks, const ttad_task_desc* task
         // GEMM/SYRK items are homed by output tile row: the items of one update share their
         // A and B panels, and the updates of one row share A, in one XCD's L2
         // spin tasks are homed like GEMM/SYRK (by their row index), so that tiny tasks spread their claims
-        if (xcd_affinity() && (x.kind == K_GEMM || x.kind == K_SYRK || x.kind == K_NOP)) home[t] = x.i % NXCD;
+        if (xcd_homed(x.kind, x.prec)) home[t] = x.i % NXCD;
     }
     // work queues in priority order; within a bucket the shared queue, then one per home XCD
     std::vector<int> qid((size_t)n_prio * (NXCD + 1), -1);
