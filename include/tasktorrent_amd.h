@@ -77,6 +77,13 @@ int ttad_init(int device, ttad_context** out);
 int ttad_finalize(ttad_context* ctx);
 int ttad_device_info(ttad_context* ctx, int* n_cu, long long* hbm_bytes, char* name, int name_len);
 int ttad_synchronize(ttad_context* ctx, void* stream);
+/* Context options (no reference counterpart).
+ * TTAD_OPT_FACTOR_RESIDUAL: value != 0 makes ttad_solve_ir and ttad_posv_mixed also compute
+ *   ||L L^T - A||_F / ||A||_F of the factor they used (ttad_factor_residual, O(n^3) work done after
+ *   the solution is complete and not counted in its times) into ttad_solve_stats.factor_residual;
+ *   0 (default) leaves that field at -1. */
+#define TTAD_OPT_FACTOR_RESIDUAL 1
+int ttad_set_option(ttad_context* ctx, int option, long long value);
 
 /* ---- tile kernels: BLAS/LAPACK-shaped, device pointers, column-major ----------- */
 /* LAPACKE_dpotrf(LAPACK_COL_MAJOR, 'L', n, A, lda)  (2d_cholesky.cpp:319).
@@ -175,8 +182,11 @@ int ttad_graph_launch(ttad_graph* g, int n_workers, int trace, void* stream);
 /* Threadpool::join (threadpool_shared.cpp:234-241): waits for completion,
  * timeout_s <= 0 waits forever. Returns NOT_SPD if a bound matrix hit a bad pivot. */
 int ttad_graph_wait(ttad_graph* g, double timeout_s, ttad_graph_stats* stats);
-/* Writes the last run's trace in the reference Logger CSV format
- * (util.cpp:117-127): "<prefix><worker>>run>NAME,start,end" in seconds. */
+/* Writes the last run's trace in the reference Logger CSV format (util.cpp:117-127, events of
+ * threadpool_shared.cpp:36-57): per task one "<prefix><worker>>run>NAME,start,end" row (first work
+ * item's start to last item's end, worker = the workgroup of its first item) and one
+ * "<prefix><worker>>deps>NAME,start,end" row (the release of its successors), times in seconds
+ * since the epoch; NAME as in ttad_graph_write_deps (POTRF_j, TRSM_i_j, GEMM_k_i_j). */
 int ttad_graph_write_trace(ttad_graph* g, const char* path, const char* prefix);
 /* Writes the graph's edges in the reference DepsLogger CSV format (util.cpp:103-112):
  * one "SRC_NAME,DST_NAME" line per fulfill_promise edge. No run needed. */
@@ -193,6 +203,8 @@ typedef struct {
     double total_s;          /* time to solution */
     int info;                /* factorization info of the factor used */
     int prec_used;           /* TTAD_F64 / TTAD_F32 / TTAD_BF16 of the factor used */
+    double factor_residual;  /* ||L L^T - A||_F / ||A||_F of the factor used, with TTAD_OPT_FACTOR_RESIDUAL;
+                                else -1 */
 } ttad_solve_stats;
 
 /* X = (L L^T)^{-1} B with a factor left by ttad_potrf_tiled (FP64 or FP32 tiles; an
@@ -206,6 +218,13 @@ int ttad_solve(ttad_context* ctx, ttad_matrix* L, int nrhs, const double* B, lon
  * (the b = A*x of 2d_cholesky.cpp:677 and its error check :682-687). */
 int ttad_residual(ttad_context* ctx, ttad_matrix* A, int nrhs, const double* B, long long ldb, const double* X,
                   long long ldx, double* R, long long ldr, double* eta, void* stream);
+/* ||L L^T - A||_F / ||A||_F over the full symmetric matrices, from the lower tiles of A (FP64) and
+ * of its factor L (FP64 or FP32 tiles, same partition; the strict upper triangle of L's diagonal
+ * tiles is ignored). FP64 products on the device, partial sums added in a fixed order.
+ * Replaces the factor check of the reference's tests (tests/shared/tests.cpp:568-583: tiled
+ * L against the monolithic factor) and complements the solve check of 2d_cholesky.cpp:646-689.
+ * Synchronous. */
+int ttad_factor_residual(ttad_context* ctx, ttad_matrix* A, ttad_matrix* L, double* rel, void* stream);
 /* Iterative refinement in FP64 with the factor L of A (any precision): X from 0,
  * r = B - A X, X += (L L^T)^{-1} r until eta <= tol or max_iter steps or a stall
  * (eta not halving twice). TTAD_ERR_IR_STALLED if tol was not reached.
@@ -236,14 +255,19 @@ typedef struct {
 } ttad_server_config;
 typedef struct {
     long long id;
-    int status;             /* TTAD_OK, TTAD_ERR_IR_STALLED, TTAD_ERR_NOT_SPD, or an admission error */
+    int status;             /* TTAD_OK, TTAD_ERR_IR_STALLED, TTAD_ERR_NOT_SPD, or an admission error. A request
+                               whose low-precision factor is not positive definite or whose refinement stalls
+                               (or reaches max_iter) is re-run with an FP64 factor before it is reported; the
+                               status is that of the FP64 attempt */
     int n, nrhs;
-    int iterations;         /* refinement steps */
+    int iterations;         /* refinement steps, all attempts */
     int reused;             /* 1 if the request used a cached factor */
     long long factor_id;    /* id of the factor used (pass as reuse_factor to solve again) */
     double backward_error;  /* normwise backward error of the returned solution */
     double latency_s;       /* submit -> solution copied back to the caller's buffer */
-    double queue_s;         /* submit -> admission */
+    double queue_s;         /* submit -> first admission */
+    int fell_back;          /* 1 if the FP64 factorization was used after the low-precision one failed */
+    int info;               /* factorization info of the factor used (1-based column of a non-positive pivot) */
 } ttad_server_result;
 typedef struct {
     int pending, active, cached_factors, workers;
@@ -252,7 +276,8 @@ typedef struct {
 
 int ttad_server_create(ttad_context* ctx, const ttad_server_config* cfg, ttad_server** out);
 /* A: FP64 tiled matrix (tile sizes multiples of 64), resident and unchanged until the
- * request completes. B: host n x nrhs column-major (copied at admission); X: host
+ * request completes. B: host n x nrhs column-major, read at admission (again if the request
+ * falls back to FP64), so it stays valid until ttad_server_poll reports the request; X: host
  * buffer that receives the solution when ttad_server_poll reports the request.
  * prec: TTAD_F64 / TTAD_F32 / TTAD_BF16 / TTAD_BF16X3 factorization; reuse_factor >= 0
  * solves with a factor kept by an earlier request (keep_factor != 0) instead of

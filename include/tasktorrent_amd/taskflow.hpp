@@ -65,6 +65,8 @@ class TaskflowBase {
     virtual int indegree(int local) = 0;
     virtual double priority(int local) = 0;
     virtual std::string task_name(int local) = 0;
+    virtual int mapping_of(int local) = 0;
+    virtual bool binding_of(int local) = 0;
 };
 
 class Threadpool {
@@ -118,6 +120,30 @@ class Threadpool {
             return TTAD_ERR_GRAPH;
         }
         const int n = (int)nodes_.size();
+        // Mapping and binding (threadpool_shared.cpp:200-224): the reference pins a task to the
+        // thread `mapping` names, and a bound task runs only on that thread, so that bound tasks of
+        // one thread never overlap (its accumulate mode binds ACCU tasks, 2d_cholesky.cpp:596-601).
+        // Device workers are not threads: every worker serves every queue. A mapping is therefore only
+        // checked (a negative thread is an error, as the reference's assert), and binding is accepted
+        // only for accumulate-mode updates (TTAD_GEMM_ACC / TTAD_SYRK_ACC), whose additions into a tile
+        // are already serialized by the tile's locks; any other bound task is refused, not ignored.
+        for (int t = 0; t < n && error_.empty(); t++) {
+            const Node& nd = nodes_[t];
+            if (nd.flow->mapping_of(nd.local) < 0) {
+                error_ = "mapping < 0 for " + nd.flow->task_name(nd.local);
+                if (verb_ > 0) fprintf(stderr, "ttad::Threadpool: %s\n", error_.c_str());
+                return TTAD_ERR_INVALID;
+            }
+            if (nd.flow->binding_of(nd.local)) {
+                const int kind = nd.flow->descriptor(nd.local).kind;
+                if (kind != TTAD_GEMM_ACC && kind != TTAD_SYRK_ACC) {
+                    error_ = "binding = true for " + nd.flow->task_name(nd.local) +
+                             ": device workers are not threads; only accumulate-mode updates may be bound";
+                    if (verb_ > 0) fprintf(stderr, "ttad::Threadpool: %s\n", error_.c_str());
+                    return TTAD_ERR_INVALID;
+                }
+            }
+        }
         desc_.assign(n, ttad_task_desc{});
         indeg_.assign(n, 0);
         out_ptr_.assign(n + 1, 0);
@@ -253,10 +279,12 @@ template <class K> class Taskflow : public TaskflowBase {
     int indegree(int local) override { return f_indegree_(keys_[local]); }
     double priority(int local) override { return f_prio_(keys_[local]); }
     std::string task_name(int local) override { return f_name_(keys_[local]); }
-    // Mapping/binding choose a CPU thread in the reference; on the device every
-    // workgroup serves every bucket, so they are kept for API parity only.
+    // Mapping/binding choose a CPU thread in the reference; Threadpool::compile checks them
+    // (the device has no threads to pin to).
     int mapping(K k) { return f_mapping_(k); }
     bool binding(K k) { return f_binding_(k); }
+    int mapping_of(int local) override { return f_mapping_(keys_[local]); }
+    bool binding_of(int local) override { return f_binding_(keys_[local]); }
 
   private:
     Threadpool* tp_;

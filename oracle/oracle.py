@@ -56,6 +56,8 @@ def lib():
         L.orc_block_sizes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip]
         L.orc_cholesky.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _dp,
                                    ctypes.c_longlong, _dp, ctypes.c_char_p, ctypes.c_char_p]
+        L.orc_priority.argtypes = [ctypes.c_int] * 7
+        L.orc_priority.restype = ctypes.c_double
         L.orc_run_dag.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _dp, _ip, _ip]
         _lib = L
     return _lib
@@ -146,6 +148,13 @@ def cholesky(A: np.ndarray, block_size: int | None = None, n_threads: int = 1, p
     stats = {k: float(v) for k, v in zip(names, st)}
     stats["info"] = int(st[6])
     return np.asfortranarray(L), stats
+
+
+def priority(prio_kind: int, nb: int, accumulate: bool, task_kind: str, i: int, j: int, k: int) -> float:
+    """Priority the restated PTG gives a task (2d_cholesky.cpp:157-204); task_kind 'POTRF', 'TRSM' or 'GEMM'
+    (SYRK is GEMM(k, i, i))."""
+    tk = {"POTRF": 0, "TRSM": 1, "GEMM": 2}[task_kind]
+    return float(lib().orc_priority(prio_kind, nb, int(accumulate), tk, i, j, k))
 
 
 def run_dag(n_threads: int, indeg, out_ptr, out_idx, prio=None):

@@ -721,6 +721,14 @@ int orc_cholesky(int n_threads, int prio_kind, int accumulate, int num_blocks, c
     return st.info;
 }
 
+// The priority run_cholesky gives a task (2d_cholesky.cpp:157-204): task_kind 0 POTRF(j), 1 TRSM(i, j),
+// 2 GEMM/SYRK(k, i, j) (SYRK when i == j), prio_kind 0..3, accumulate as the --accumulate flag.
+double orc_priority(int prio_kind, int nb, int accumulate, int task_kind, int i, int j, int k) {
+    if (task_kind == 0) return orc::prio_potrf(prio_kind, nb, j);
+    if (task_kind == 1) return orc::prio_trsm(prio_kind, nb, i, j);
+    return orc::prio_gemm(prio_kind, nb, accumulate, k, i, j);
+}
+
 // Scheduler check on an arbitrary DAG given in CSR form (tests/shared/tests.cpp:294-358):
 // every task runs once, after all its predecessors; order[] receives the completion order.
 // Task t has indegree indeg[t]; tasks with indeg 0 are seeded by the main thread.

@@ -18,13 +18,13 @@ import numpy as np
 from . import _capi
 from ._capi import (TTAD_ACCUMULATE, TTAD_BF16, TTAD_BF16X3, TTAD_F32, TTAD_F64, TTAD_GEMM, TTAD_GEMM_ACC,
                     TTAD_POTRF, TTAD_SCHED_PERSISTENT, TTAD_SCHED_STREAM, TTAD_SPIN, TTAD_SYRK, TTAD_SYRK_ACC,
-                    TTAD_TRSM, TtadError, lib, lib_path)
+                    TTAD_TRSM, TTAD_ERR_IR_STALLED, TTAD_ERR_NOT_SPD, TtadError, lib, lib_path)
 
 __all__ = [
     "Context", "TiledMatrix", "Graph", "Server", "TtadError", "lib", "lib_path",
     "TTAD_F64", "TTAD_F32", "TTAD_BF16", "TTAD_BF16X3", "TTAD_POTRF", "TTAD_TRSM", "TTAD_SYRK", "TTAD_GEMM", "TTAD_SPIN",
     "TTAD_GEMM_ACC", "TTAD_SYRK_ACC", "TTAD_ACCUMULATE",
-    "TTAD_SCHED_STREAM", "TTAD_SCHED_PERSISTENT",
+    "TTAD_SCHED_STREAM", "TTAD_SCHED_PERSISTENT", "TTAD_ERR_NOT_SPD", "TTAD_ERR_IR_STALLED",
 ]
 
 PREC = {"f64": TTAD_F64, "f32": TTAD_F32, "bf16": TTAD_BF16, "bf16x3": TTAD_BF16X3}
@@ -135,7 +135,8 @@ class Context:
     @staticmethod
     def _stats(rc, st):
         return {"rc": rc, "iterations": st.iterations, "fell_back": st.fell_back,
-                "backward_error": st.backward_error, "factor_s": st.factor_s, "solve_s": st.solve_s,
+                "backward_error": st.backward_error, "factor_residual": st.factor_residual,
+                "factor_s": st.factor_s, "solve_s": st.solve_s,
                 "total_s": st.total_s, "info": st.info, "prec_used": st.prec_used}
 
     def solve_ir(self, A: "TiledMatrix", L: "TiledMatrix", nrhs: int, B, X, tol=1e-12, max_iter=30, stream=None):
@@ -157,6 +158,19 @@ class Context:
             _capi.check(rc)
         self.synchronize(stream)
         return self._stats(rc, st)
+
+    def factor_residual(self, A: "TiledMatrix", L: "TiledMatrix", stream=None) -> float:
+        """||L L^T - A||_F / ||A||_F on the device (ttad_factor_residual)."""
+        r = ctypes.c_double()
+        _capi.check(lib().ttad_factor_residual(self._h, A.handle, L.handle, ctypes.byref(r), stream))
+        return r.value
+
+    def set_option(self, option: int, value: int) -> None:
+        _capi.check(lib().ttad_set_option(self._h, option, value))
+
+    def report_factor_residual(self, on: bool = True) -> None:
+        """ttad_solve_ir / ttad_posv_mixed also report ||L L^T - A||_F / ||A||_F (untimed, O(n^3))."""
+        self.set_option(_capi.TTAD_OPT_FACTOR_RESIDUAL, int(on))
 
     def last_elapsed(self) -> float:
         """Device seconds of the last potrf_tiled (HIP events on its stream)."""

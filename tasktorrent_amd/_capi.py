@@ -19,6 +19,7 @@ TTAD_ERR_IR_STALLED = 6
 TTAD_ERR_GRAPH = 7
 TTAD_ERR_BUSY = 8
 TTAD_ERR_NOT_FOUND = 9
+TTAD_OPT_FACTOR_RESIDUAL = 1
 
 TTAD_F64, TTAD_F32, TTAD_BF16, TTAD_BF16X3 = 0, 1, 2, 3
 TTAD_POTRF, TTAD_TRSM, TTAD_SYRK, TTAD_GEMM = 0, 1, 2, 3
@@ -50,7 +51,7 @@ class GraphStats(ctypes.Structure):
 class SolveStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("fell_back", ctypes.c_int), ("backward_error", ctypes.c_double),
                 ("factor_s", ctypes.c_double), ("solve_s", ctypes.c_double), ("total_s", ctypes.c_double),
-                ("info", ctypes.c_int), ("prec_used", ctypes.c_int)]
+                ("info", ctypes.c_int), ("prec_used", ctypes.c_int), ("factor_residual", ctypes.c_double)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -61,7 +62,8 @@ class ServerConfig(ctypes.Structure):
 class ServerResult(ctypes.Structure):
     _fields_ = [("id", ctypes.c_longlong), ("status", ctypes.c_int), ("n", ctypes.c_int), ("nrhs", ctypes.c_int),
                 ("iterations", ctypes.c_int), ("reused", ctypes.c_int), ("factor_id", ctypes.c_longlong),
-                ("backward_error", ctypes.c_double), ("latency_s", ctypes.c_double), ("queue_s", ctypes.c_double)]
+                ("backward_error", ctypes.c_double), ("latency_s", ctypes.c_double), ("queue_s", ctypes.c_double),
+                ("fell_back", ctypes.c_int), ("info", ctypes.c_int)]
 
 
 class ServerStats(ctypes.Structure):
@@ -86,6 +88,8 @@ _SIGS = {
     "ttad_finalize": (I, [V]),
     "ttad_device_info": (I, [V, PI, ctypes.POINTER(LL), ctypes.c_char_p, I]),
     "ttad_synchronize": (I, [V, V]),
+    "ttad_set_option": (I, [V, I, LL]),
+    "ttad_factor_residual": (I, [V, V, V, ctypes.POINTER(D), V]),
     "ttad_dpotrf": (I, [V, I, V, I, PI, V]),
     "ttad_spotrf": (I, [V, I, V, I, PI, V]),
     "ttad_dtrsm": (I, [V, I, I, V, I, V, I, V]),

@@ -14,7 +14,7 @@ PKG = Path(__file__).resolve().parent
 SRC = PKG / "csrc"
 OUT_DIR = PKG / "_lib"
 LIB = OUT_DIR / "libtasktorrent_amd.so"
-SOURCES = ["kernels.hip", "capi.hip", "dense_cholesky.hip", "solve.hip", "server.hip"]
+SOURCES = ["kernels.hip", "capi.hip", "dense_cholesky.hip", "solve.hip", "server.hip", "factor_check.hip"]
 ARCH = os.environ.get("TTAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -132,6 +132,7 @@ int ttad_init(int device, ttad_context** out) {
     c->name = p.gcnArchName;
     hipError_t e = hipStreamCreate(&c->stream);
     if (e == hipSuccess) e = kernels_init();
+    if (e == hipSuccess) e = factor_check_init();
     if (e == hipSuccess) e = c->scratch_info.alloc(sizeof(int));
     if (e != hipSuccess) {
         delete c;
@@ -181,6 +182,17 @@ int ttad_synchronize(ttad_context* ctx, void* stream) {
     if (!ctx) return set_error(TTAD_ERR_INVALID, "ctx is NULL");
     TTAD_HIP(hipStreamSynchronize(pick(ctx, stream)));
     return TTAD_OK;
+}
+
+int ttad_set_option(ttad_context* ctx, int option, long long value) {
+    if (!ctx) return set_error(TTAD_ERR_INVALID, "ctx is NULL");
+    switch (option) {
+        case TTAD_OPT_FACTOR_RESIDUAL:
+            ctx->report_factor_residual = value != 0;
+            return TTAD_OK;
+        default:
+            return set_error(TTAD_ERR_INVALID, "set_option: unknown option %d", option);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -712,11 +724,17 @@ static int graph_prepare(ttad_graph* g) {
             if (x.kind != K_NOP) needed |= item_variant(x.kind, x.prec, g->mats[x.mat]->storage) | (x.accum ? V_ACC : 0);
         g->vset = vset_index(needed);
         g->bucket_items.assign(g->n_buckets, 0);
-        g->total_items = 0;
+        // queue entries hold (task << 20) | item in 64 bits and slots are 32-bit indices
+        long long total = 0;
         for (auto& x : g->tasks) {
-            g->bucket_items[x.bucket] += x.n_items;
-            g->total_items += x.n_items;
+            if (x.n_items >= (1 << 20))
+                return set_error(TTAD_ERR_INVALID, "graph: a task has %d work items (limit 2^20)", x.n_items);
+            total += x.n_items;
         }
+        if (total >= (1LL << 31) - 1)
+            return set_error(TTAD_ERR_INVALID, "graph: %lld work items in all (limit 2^31 - 1)", total);
+        g->total_items = (int)total;
+        for (auto& x : g->tasks) g->bucket_items[x.bucket] += x.n_items;
         g->q_base.assign(g->n_buckets + 1, 0);
         for (int b = 0; b < g->n_buckets; b++) g->q_base[b + 1] = g->q_base[b] + g->bucket_items[b];
         std::vector<DevMat> dm(g->mats.size());
@@ -761,21 +779,24 @@ static int graph_prepare(ttad_graph* g) {
                 return set_error(TTAD_ERR_GRAPH, "graph: task %d has indegree %d but receives %d edges + %d seeds", t,
                                  g->indegree[t], incoming[t], g->seeds[t]);
         std::vector<int32_t> deps0(n), tail0(g->n_buckets, 0);
-        std::vector<int64_t> init;
+        std::vector<int64_t> init;  // queue entries (task << 20) | item, as the device pushes them
+        std::vector<int32_t> init_slot;
         for (int t = 0; t < n; t++) {
             deps0[t] = g->indegree[t] - g->seeds[t];
             if (deps0[t] == 0) {
                 const DevTask& x = g->tasks[t];
                 for (int r = 0; r < x.n_items; r++) {
-                    const int64_t slot = g->q_base[x.bucket] + tail0[x.bucket]++;
-                    init.push_back((slot << 40) | ((int64_t)t << 20) | r);
+                    init_slot.push_back(g->q_base[x.bucket] + tail0[x.bucket]++);
+                    init.push_back(((int64_t)t << 20) | r);
                 }
             }
         }
         if (init.empty()) return set_error(TTAD_ERR_GRAPH, "graph: no task is ready (seed one with ttad_graph_seed)");
         g->n_init = (int)init.size();
         TTAD_HIP(g->d_init.alloc(init.size() * 8));
+        TTAD_HIP(g->d_init_slot.alloc(init_slot.size() * 4));
         TTAD_HIP(hipMemcpy(g->d_init.p, init.data(), init.size() * 8, hipMemcpyHostToDevice));
+        TTAD_HIP(hipMemcpy(g->d_init_slot.p, init_slot.data(), init_slot.size() * 4, hipMemcpyHostToDevice));
         TTAD_HIP(hipMemcpy(g->d_deps0.p, deps0.data(), n * 4, hipMemcpyHostToDevice));
         TTAD_HIP(hipMemcpy(g->d_tail0.p, tail0.data(), g->n_buckets * 4, hipMemcpyHostToDevice));
         g->seeds_dirty = false;
@@ -795,6 +816,7 @@ static DevGraph graph_image(ttad_graph* g, bool trace, int64_t spin_limit) {
     d.deps0 = g->d_deps0.as<int32_t>();
     d.tail0 = g->d_tail0.as<int32_t>();
     d.init_items = g->d_init.as<int64_t>();
+    d.init_slots = g->d_init_slot.as<int32_t>();
     d.deps = g->d_deps.as<int32_t>();
     d.items_left = g->d_items_left.as<int32_t>();
     d.q_head = g->d_q_head.as<int32_t>();
@@ -822,7 +844,7 @@ int ttad_graph_launch(ttad_graph* g, int n_workers, int trace, void* stream) {
     ttad_context* ctx = g->ctx;
     int rc = graph_prepare(g);
     if (rc) return rc;
-    if (trace) TTAD_HIP(g->d_trace.alloc((size_t)g->total_items * 4 * 8));
+    if (trace) TTAD_HIP(g->d_trace.alloc((size_t)g->total_items * TRACE_WORDS * 8));
     g->traced = trace != 0;
     int rate_khz = 100000;
     (void)hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device);
@@ -908,27 +930,68 @@ static std::string task_label(const DevTask& t) {
 int ttad_graph_write_trace(ttad_graph* g, const char* path, const char* prefix) {
     if (!g || !path) return set_error(TTAD_ERR_INVALID, "graph_write_trace: bad args");
     if (!g->traced || g->running) return set_error(TTAD_ERR_INVALID, "graph_write_trace: no finished traced run");
-    std::vector<uint64_t> rec((size_t)g->total_items * 4);
+    constexpr int W = TRACE_WORDS;
+    std::vector<uint64_t> rec((size_t)g->total_items * W);
     int cnt = 0;
     TTAD_HIP(hipMemcpy(&cnt, g->d_trace_cursor.p, 4, hipMemcpyDeviceToHost));
     cnt = std::min(cnt, g->total_items);
-    TTAD_HIP(hipMemcpy(rec.data(), g->d_trace.p, (size_t)cnt * 32, hipMemcpyDeviceToHost));
+    TTAD_HIP(hipMemcpy(rec.data(), g->d_trace.p, (size_t)cnt * W * 8, hipMemcpyDeviceToHost));
     int rate_khz = 100000;
     (void)hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, g->ctx->device);
-    uint64_t t0 = ~0ull;
-    for (int r = 0; r < cnt; r++) t0 = std::min(t0, rec[4 * r + 2]);
+    const double tick = 1.0 / ((double)rate_khz * 1000.0);
+    // Device clock -> seconds since the epoch, as the reference's Logger writes times (util.cpp:18-20, 117-127):
+    // read the device clock between two host clock readings and take the midpoint.
+    DevBuf clk;
+    TTAD_HIP(clk.alloc(8));
+    TTAD_HIP(hipStreamSynchronize(g->ctx->stream));
+    const double h0 = std::chrono::duration<double>(std::chrono::system_clock::now().time_since_epoch()).count();
+    launch_read_clock(clk.as<uint64_t>(), g->ctx->stream);
+    TTAD_HIP(hipStreamSynchronize(g->ctx->stream));
+    const double h1 = std::chrono::duration<double>(std::chrono::system_clock::now().time_since_epoch()).count();
+    uint64_t dclk = 0;
+    TTAD_HIP(hipMemcpy(&dclk, clk.p, 8, hipMemcpyDeviceToHost));
+    const double origin = 0.5 * (h0 + h1);  // epoch seconds at device clock dclk
+    auto epoch = [&](uint64_t c) { return origin + ((double)(int64_t)(c - dclk)) * tick; };
+    // One ">run>" row per task, as a Threadpool worker logs it (threadpool_shared.cpp:36-57): from the start
+    // of its first work item to the end of its last, on the workgroup that ran its first item; one ">deps>" row
+    // for the release of its successors (the fulfill step), on the workgroup that ran its last item.
+    struct Span {
+        uint64_t start = ~0ull, end = 0, deps_start = 0, deps_end = 0;
+        uint64_t who_run = 0, who_deps = 0;
+        int items = 0;
+    };
+    std::vector<Span> span(g->n_tasks);
+    for (int r = 0; r < cnt; r++) {
+        const uint64_t* x = rec.data() + (size_t)r * W;
+        const size_t t = (size_t)(x[0] >> 20);
+        if (t >= span.size()) continue;
+        Span& s = span[t];
+        if (x[2] < s.start) {
+            s.start = x[2];
+            s.who_run = x[1];
+        }
+        if (x[3] > s.end) s.end = x[3];
+        if (x[4]) {
+            s.deps_start = x[3];
+            s.deps_end = x[4];
+            s.who_deps = x[1];
+        }
+        s.items++;
+    }
     std::ofstream f(path);
     if (!f) return set_error(TTAD_ERR_INVALID, "graph_write_trace: cannot open %s", path);
-    f.precision(12);
-    const double tick = 1.0 / ((double)rate_khz * 1000.0);
+    f.precision(17);
     const std::string pre = prefix ? prefix : "Wk_";
-    for (int r = 0; r < cnt; r++) {
-        const uint64_t e = rec[4 * r];
-        const DevTask& t = g->tasks[(size_t)(e >> 20)];
-        f << pre << rec[4 * r + 1] << ">run>" << task_label(t) << "#" << (e & 0xFFFFF) << ","
-          << (double)(rec[4 * r + 2] - t0) * tick << "," << (double)(rec[4 * r + 3] - t0) * tick << "\n";
+    for (int t = 0; t < g->n_tasks; t++) {
+        const Span& s = span[t];
+        if (s.items == 0) continue;
+        const std::string name = task_label(g->tasks[t]);
+        f << pre << s.who_run << ">run>" << name << "," << epoch(s.start) << "," << epoch(s.end) << "\n";
+        if (s.deps_end)
+            f << pre << s.who_deps << ">deps>" << name << "," << epoch(s.deps_start) << "," << epoch(s.deps_end)
+              << "\n";
     }
-    return TTAD_OK;
+    return f ? TTAD_OK : set_error(TTAD_ERR_INVALID, "graph_write_trace: write failed");
 }
 
 int ttad_graph_write_deps(ttad_graph* g, const char* path) {

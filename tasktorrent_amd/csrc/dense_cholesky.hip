@@ -28,7 +28,10 @@ static double prio_trsm(int kind, int nb, int i, int j) {
     if (kind == 1) return 2.0 * (double)(nb - i);
     return 2.0;
 }
-static double prio_gemm(int kind, int nb, int k, int i, int j) {
+// accumulate: the accumulate-mode formula of kind 3 (2d_cholesky.cpp:187-189), which ranks the
+// updates of one tile alike since they no longer form a chain in k.
+static double prio_gemm(int kind, int nb, int k, int i, int j, bool accumulate) {
+    if (kind == 3 && accumulate) return (double)(nb - i) + nb * (9.0 * nb - 9.0 * j - 2.0);
     if (kind == 3) return (double)(nb - i) + nb * (9.0 * nb - 3.0 * j - 6.0 * k - 2.0);
     if (kind == 2) return (double)(9 * nb - 9 * j - 2);
     if (kind == 1) return (double)(nb - i);
@@ -118,7 +121,7 @@ static int build_cholesky_graph(ttad_context* ctx, int nb, int storage, int prec
             return (kij[1] == kij[2] ? 1 : 2) + (acc || kij[0] == 0 ? 0 : 1);
         })
         .set_priority([&](int3 kij) {
-            return bl ? bl->gemm(kij[0], kij[1], kij[2]) : prio_gemm(prio_kind, nb, kij[0], kij[1], kij[2]);
+            return bl ? bl->gemm(kij[0], kij[1], kij[2]) : prio_gemm(prio_kind, nb, kij[0], kij[1], kij[2], acc);
         })
         .set_mapping([&](int3 kij) { return block_2_thread(kij[1], kij[2]); })
         .set_binding([](int3) { return false; })

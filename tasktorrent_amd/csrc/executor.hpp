@@ -18,7 +18,8 @@ struct DevGraph {
     const int32_t* out_idx;
     const int32_t* deps0;       // initial dependency counters (indegree - host seeds)
     const int32_t* tail0;       // initial queue tails per bucket
-    const int64_t* init_items;  // [n_init] (bucket << 40) | entry for prefilled slots
+    const int64_t* init_items;  // [n_init] queue entries (task << 20) | item of the prefilled slots
+    const int32_t* init_slots;  // [n_init] their slots in q_items
     int32_t* deps;
     int32_t* items_left;
     int32_t* q_head;            // [n_buckets]
@@ -31,7 +32,8 @@ struct DevGraph {
     int32_t* done_items;
     int32_t* abort_flag;        // 1 = watchdog, 2 = queue overflow
     const DevMat* mats;
-    uint64_t* trace;            // optional: 4 x u64 per item {task<<20|item, wg, t_start, t_end}
+    uint64_t* trace;            // optional: TRACE_WORDS x u64 per item {task<<20|item, wg, t_start, t_end,
+                                // t_end of the successor release (the task's last item; else 0), 0}
     int32_t* trace_cursor;
     int32_t* cu_slots;          // [CU_KEYS] workers started per CU (slot 0 = primary)
     int32_t* idle_primary;      // primary workers currently looking for work
@@ -39,6 +41,8 @@ struct DevGraph {
     int32_t* xcd_run;           // [n_tasks * NXCD] items of a task running on each XCD (release elision)
     int64_t spin_limit;         // watchdog in wall_clock64 ticks
 };
+
+constexpr int TRACE_WORDS = 6;  // u64 per trace record
 
 // XCDs of the device (MI355X: 8 XCDs of 32 CUs, each with its own L2).
 constexpr int NXCD = 8;

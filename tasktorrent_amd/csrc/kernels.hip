@@ -183,27 +183,25 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
         uint64_t t0 = 0;
         if (g.trace && tid == 0) t0 = wall_clock64();
         const bool ran = exec_item<VM>(t, item, g.mats, lds, g.abort_flag, g.spin_limit);
+        // Every wave waits until the L2 has acknowledged its own stores (gfx9: vmcnt counts stores too;
+        // encoding vmcnt(0) expcnt(7) lgkmcnt(15)) before the barrier: after it, thread 0's decrement
+        // below is issued only once all of this item's outputs sit in this XCD's L2.
+        __builtin_amdgcn_s_waitcnt(0x0F70);
         __syncthreads();
         if (tid == 0) {
-            // Release elision: an agent-scope release writes back every dirty line of this XCD's L2,
-            // so the item that leaves no other item of its task running on this XCD releases for all
-            // of them (their stores completed before their own decrement, which came first); the
-            // others skip it. Each item decrements the task's remaining count after that, so the
-            // count reaches zero only after every XCD that ran items of the task has released.
+            // Release elision: an agent-scope release (buffer_wbl2) writes back every dirty line of this
+            // XCD's L2, so the item that leaves no other item of its task running on this XCD releases for
+            // all of them; the others skip it. This relies on the hardware, not on the memory model: an
+            // item's stores reached the L2 before its relaxed decrement was issued (the wait above), the
+            // decrements of one counter are serialized at the L2, and the last one's write-back is issued
+            // after its decrement returned, so it covers every earlier item's lines. Each item decrements
+            // the task's remaining count after that, so the count reaches zero only after every XCD that
+            // ran items of the task has released.
             if (__hip_atomic_fetch_add(g.xcd_run + (long)task * NXCD + my_xcd, -1, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT) == 1)
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             if (!ran) __hip_atomic_store(g.abort_flag, 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (g.trace) {
-                const int slot = __hip_atomic_fetch_add(g.trace_cursor, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (slot < g.total_items) {
-                    uint64_t* rec = g.trace + 4 * (long)slot;
-                    rec[0] = (uint64_t)e;
-                    rec[1] = blockIdx.x;
-                    rec[2] = t0;
-                    rec[3] = wall_clock64();
-                }
-            }
+            const uint64_t t1 = g.trace ? wall_clock64() : 0;
             // Counters are relaxed: the release fence above publishes this item's outputs. The thread
             // that takes a counter to zero acquires what the other decrements released and releases it again
             // before its own next counter update or queue entries (one acq_rel fence per task, and per
@@ -222,6 +220,18 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
                     }
                 }
                 __hip_atomic_fetch_add(g.done_items, t.n_items, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (g.trace) {  // {entry, workgroup, start, end, end of the successor release (last item only), 0}
+                const int slot = __hip_atomic_fetch_add(g.trace_cursor, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (slot < g.total_items) {
+                    uint64_t* rec = g.trace + TRACE_WORDS * (long)slot;
+                    rec[0] = (uint64_t)e;
+                    rec[1] = blockIdx.x;
+                    rec[2] = t0;
+                    rec[3] = t1;
+                    rec[4] = left == 0 ? wall_clock64() : 0;
+                    rec[5] = 0;
+                }
             }
             if (primary) __hip_atomic_fetch_add(g.idle_primary, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (t.crit) __hip_atomic_fetch_add(g.cu_crit + my_key, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -257,9 +267,7 @@ __global__ void k_graph_reset_a(DevGraph g, int n_slots) {
 __global__ void k_graph_reset_b(DevGraph g) {
     const int stride = gridDim.x * blockDim.x;
     for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < g.n_init; x += stride) {
-        const int64_t v = g.init_items[x];
-        const int64_t slot = v >> 40;
-        g.q_items[slot] = v & ((1LL << 40) - 1);
+        g.q_items[g.init_slots[x]] = g.init_items[x];
     }
 }
 
@@ -455,6 +463,11 @@ void launch_items(int vset, const DevMat* mats, const DevTask* tasks, const int3
         default: hipLaunchKernelGGL(k_items<VSETS[4]>, grid, block, lds, s, mats, tasks, prefix, n_tasks, err); break;
     }
 }
+
+__global__ void k_read_clock(uint64_t* out) {
+    if (threadIdx.x == 0) *out = wall_clock64();
+}
+void launch_read_clock(uint64_t* out, hipStream_t s) { hipLaunchKernelGGL(k_read_clock, dim3(1), dim3(64), 0, s, out); }
 
 void launch_graph_reset(const DevGraph& g, int n_slots, hipStream_t s) {
     hipLaunchKernelGGL(k_graph_reset_a, dim3(256), dim3(256), 0, s, g, n_slots);

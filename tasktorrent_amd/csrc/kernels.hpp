@@ -14,6 +14,8 @@ int persistent_blocks_per_cu(int vset);
 void launch_items(int vset, const DevMat* mats, const DevTask* tasks, const int32_t* prefix, int n_tasks,
                   int n_items, int32_t* err, hipStream_t s);
 void launch_graph_reset(const DevGraph& g, int n_slots, hipStream_t s);
+// Writes the device wall clock (wall_clock64) to *out.
+void launch_read_clock(uint64_t* out, hipStream_t s);
 void launch_persistent(int vset, const DevGraph& g, int n_workers, hipStream_t s);
 void launch_raw_gemm(int prec, const void* A, long lda, const void* B, long ldb, void* C, long ldc, int m, int n,
                      int K, int syrk, double alpha, double beta, hipStream_t s);

@@ -17,6 +17,9 @@
 namespace ttad {
 
 int set_error(int code, const char* fmt, ...);
+hipError_t factor_check_init();
+// ||L L^T - A||_F / ||A||_F (factor_check.hip); synchronizes s.
+int factor_residual(ttad_context* ctx, ttad_matrix* A, ttad_matrix* L, double* rel, hipStream_t s);
 
 #define TTAD_HIP(call)                                                                                   \
     do {                                                                                                 \
@@ -72,6 +75,8 @@ struct ttad_context {
     double kernel_s = -1.0;  // event time of the last persistent factorization kernel (< 0: stream schedule)
     // solve-phase workspace (grown on demand)
     ttad::DevBuf sw_Y, sw_D, sw_R, sw_flags, sw_ticket, sw_err, sw_norms, sw_part;
+    // options (ttad_set_option)
+    bool report_factor_residual = false;
     // working copies of FP64 matrices for ttad_posv_mixed, keyed by (matrix id, storage)
     std::map<std::pair<long long, int>, ttad_matrix*> work_mats;
 };
@@ -102,7 +107,7 @@ struct ttad_graph {
     std::vector<ttad_matrix*> mats;
     std::vector<int32_t> spin_us;  // per task: spin time of a TTAD_SPIN task (microseconds)
     bool seeds_dirty = true, mats_dirty = true, running = false;
-    ttad::DevBuf d_tasks, d_out_ptr, d_out_idx, d_deps0, d_tail0, d_init, d_deps, d_items_left, d_q_head, d_q_tail, d_q_count, d_xcd_run,
+    ttad::DevBuf d_tasks, d_out_ptr, d_out_idx, d_deps0, d_tail0, d_init, d_init_slot, d_deps, d_items_left, d_q_head, d_q_tail, d_q_count, d_xcd_run,
         d_q_base, d_q_home, d_q_items, d_done, d_abort, d_mats, d_trace, d_trace_cursor, d_cu_slots;
     int n_init = 0;
     int workers = 0;

@@ -244,6 +244,9 @@ struct ttad_server {
         long long reuse;   // factor id to reuse, -1 = factor A
         int keep;          // keep the factor after completion
         double t_submit;   // host seconds
+        int fell_back = 0;    // this is the FP64 re-run of a request whose low-precision attempt failed
+        int prior_iters = 0;  // refinement steps of that attempt
+        double t_admit0 = -1; // first admission (host seconds)
     };
     struct Active {
         Pending p;
@@ -891,6 +894,8 @@ static int server_pump(ttad_server* S) {
             ttad_server_result r{};
             r.id = S->pending.front().id;
             r.status = rc;
+            r.fell_back = S->pending.front().fell_back;
+            r.iterations = S->pending.front().prior_iters;
             S->results.push_back(r);
             S->pending.pop_front();
             continue;
@@ -967,18 +972,35 @@ int ttad_server_poll(ttad_server* S, ttad_server_result* out, int max_out, int* 
     for (long long id : finished) {
         auto& a = S->active[id];
         const SrvDone d = S->done[a.slot];
+        if (d.state != SRV_OK && a.p.prec != P_F64 && !a.p.fell_back) {
+            // Automatic FP64 fallback: the low-precision factor was not positive definite or refinement
+            // stalled. The request goes back to the head of the admission queue with an FP64 factor of A
+            // (a reused low-precision factor is dropped for this request) and is reported when that ends.
+            ttad_server::Pending p = a.p;
+            p.prec = P_F64;
+            p.reuse = -1;
+            p.fell_back = 1;
+            p.prior_iters = d.iterations;
+            p.t_admit0 = a.t_admit;
+            server_release(S, a, false);
+            S->active.erase(id);
+            S->pending.push_front(p);
+            continue;
+        }
         ttad_server_result r{};
         r.id = id;
         r.n = a.p.A->n;
         r.nrhs = a.p.nrhs;
-        r.iterations = d.iterations;
+        r.iterations = a.p.prior_iters + d.iterations;
+        r.fell_back = a.p.fell_back;
+        r.info = d.info;
         r.backward_error = d.eta;
         r.factor_id = a.factor;
         r.reused = a.p.reuse >= 0;
         r.status = d.state == SRV_OK ? TTAD_OK : (d.state == SRV_NOT_SPD ? TTAD_ERR_NOT_SPD : TTAD_ERR_IR_STALLED);
         memcpy(a.p.X, S->rbuf + a.rbuf_off, (size_t)r.n * r.nrhs * 8);  // solution written by the device
         r.latency_s = S->now() - a.p.t_submit;
-        r.queue_s = a.t_admit - a.p.t_submit;
+        r.queue_s = (a.p.t_admit0 >= 0 ? a.p.t_admit0 : a.t_admit) - a.p.t_submit;
         const bool keep = a.p.keep && r.status == TTAD_OK;
         server_release(S, a, keep);
         S->active.erase(id);

@@ -329,6 +329,11 @@ int ttad_solve_ir(ttad_context* ctx, ttad_matrix* A, ttad_matrix* L, int nrhs, c
         st->iterations = iters;
         st->backward_error = eta;
         st->prec_used = L->storage;
+        st->factor_residual = -1.0;
+        if (ctx->report_factor_residual && (rc == TTAD_OK || rc == TTAD_ERR_IR_STALLED)) {
+            const int r2 = factor_residual(ctx, A, L, &st->factor_residual, s);
+            if (r2 != TTAD_OK) return r2;
+        }
     }
     return rc;
 }
@@ -344,6 +349,7 @@ int ttad_posv_mixed(ttad_context* ctx, ttad_matrix* A, int prec, int nrhs, const
     int rc = TTAD_OK;
     ttad_solve_stats loc{};
     loc.prec_used = prec;
+    loc.factor_residual = -1.0;
     ttad_matrix* W = working(ctx, A, prec == P_F64 ? P_F64 : P_F32, rc);
     if (!W) return rc;
     if ((rc = ttad_matrix_copy(W, A, s))) return rc;
@@ -375,10 +381,16 @@ int ttad_posv_mixed(ttad_context* ctx, ttad_matrix* A, int prec, int nrhs, const
             loc.backward_error = eta;
         }
         loc.prec_used = P_F64;
+        W = W64;
     }
     TTAD_HIP(hipStreamSynchronize(s));
     loc.total_s = secs();
     loc.solve_s = loc.total_s - loc.factor_s;
+    // the factor residual is reported after the solution is complete and is not part of its times
+    if (ctx->report_factor_residual && (rc == TTAD_OK || rc == TTAD_ERR_IR_STALLED)) {
+        const int r2 = factor_residual(ctx, A, W, &loc.factor_residual, s);
+        if (r2 != TTAD_OK) return r2;
+    }
     if (st) *st = loc;
     return rc;
 }

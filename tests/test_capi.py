@@ -77,6 +77,35 @@ def test_dag_priorities_match_prio_kind():
                 assert p == {tt.TTAD_POTRF: 3.0, tt.TTAD_TRSM: 2.0}.get(k_, 1.0)
 
 
+@pytest.mark.parametrize("accumulate", [False, True])
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+@pytest.mark.parametrize("nb", [1, 4, 9])
+def test_dag_priorities_equal_oracle(oracle, nb, kind, accumulate):
+    """Every task's priority, every PrioKind, chained and accumulate mode, against the oracle's PTG."""
+    flag = tt.TTAD_ACCUMULATE if accumulate else 0
+    tasks, _, prio, _, _ = tt.cholesky_dag(nb, kind | flag)
+    for t, p in zip(tasks, prio):
+        name = tt.task_name(t).split("_")[0]
+        _, _, _, i, j, k = t
+        assert p == oracle.priority(kind, nb, accumulate, name, i, j, k), (kind, accumulate, t, p)
+
+
+def test_dag_priorities_kind_1_and_3_values():
+    """Hand-evaluated PrioKind 1 (row) and 3 (cp_row) values of 2d_cholesky.cpp:157-204 at nb = 4."""
+    nb = 4
+    for kind, acc, want in [
+        (1, False, {"POTRF_1": 9.0, "TRSM_3_1": 2.0, "GEMM_0_2_1": 2.0, "GEMM_1_3_3": 1.0}),
+        (3, False, {"POTRF_1": 26.0 + 288.0, "TRSM_3_1": 1 + 4 * 25 + 144.0, "GEMM_0_2_1": 2 + 4 * 31.0,
+                    "GEMM_1_3_3": 1 + 4 * 19.0}),
+        # accumulate mode drops the k term of the GEMM priority: 9nb - 9j - 2
+        (3, True, {"GEMM_0_2_1": 2 + 4 * 25.0, "GEMM_1_3_3": 1 + 4 * 7.0}),
+    ]:
+        tasks, _, prio, _, _ = tt.cholesky_dag(nb, kind | (tt.TTAD_ACCUMULATE if acc else 0))
+        got = {tt.task_name(t): p for t, p in zip(tasks, prio)}
+        for name, v in want.items():
+            assert got[name] == v, (kind, acc, name, got[name], v)
+
+
 @pytest.mark.parametrize("nb", [1, 2, 3, 6, 11])
 def test_dag_bottom_level_priorities(nb):
     # prio_kind 4: longest path to the end of the DAG with weights POTRF 2, TRSM 2, GEMM/SYRK 1,

@@ -9,8 +9,10 @@ a claimed slot is always below the tail, no slot is claimed twice, every pushed 
 
 Release elision: an agent-scope release writes back every dirty line of one XCD's L2, so for each
 task only the item that takes the task's running-item counter of its XCD to zero releases; every
-item then decrements the task's remaining count. Checked: when that count reaches zero, no store
-of any item of the task is still held in an L2 (all are visible to the successors).
+item then decrements the task's remaining count. Stores travel to the L2 asynchronously; each wave
+waits for its own before the post-item barrier. Checked: when that count reaches zero, no store of
+any item of the task is still held in an L2 or in flight (all are visible to the successors), and
+the same model without the store wait does lose stores.
 """
 import random
 
@@ -76,20 +78,32 @@ def test_claim_credits_never_pass_the_tail(seed):
     _claims(seed)
 
 
-def _elision(seed, n_items=16, n_xcd=4):
+def _elision(seed, n_items=16, n_xcd=4, wait=True):
+    """One task's items on random XCDs. A store is issued into flight first and reaches its XCD's L2
+    at a later step of the scheduler's choosing; `wait` models the per-wave store wait before the
+    post-item barrier (s_waitcnt vmcnt(0)), after which the item's stores are in the L2 before its
+    relaxed decrement. Returns True when every store was written back before the task's count hit 0."""
     rnd = random.Random(seed)
     run = [0] * n_xcd
     dirty = [set() for _ in range(n_xcd)]  # stores held in each XCD's L2
     visible = set()
     left = [n_items]
     done_check = []
+    in_flight = {}  # item -> XCD its stores are travelling to
+
+    def deliver(it):
+        if it in in_flight:
+            dirty[in_flight.pop(it)].add(it)
 
     def item(it):
         x = rnd.randrange(n_xcd)
         run[x] += 1  # start
         yield
-        dirty[x].add(it)  # the item's stores (complete before its decrement)
+        in_flight[it] = x  # the item's stores are issued
         yield
+        if wait:  # every wave waits for its stores to be acknowledged by the L2
+            deliver(it)
+            yield
         run[x] -= 1
         last_here = run[x] == 0
         yield
@@ -102,17 +116,35 @@ def _elision(seed, n_items=16, n_xcd=4):
             done_check.append(set(range(n_items)) <= visible)
         yield
 
+    def memory():  # stores in flight reach the L2 in any order, at any time
+        while True:
+            if in_flight:
+                deliver(rnd.choice(sorted(in_flight)))
+            yield
+
     actors = [item(i) for i in range(n_items)]
+    mem = memory()
     live = list(range(n_items))
     while live:
+        if rnd.random() < 0.3:
+            next(mem)
+            continue
         a = rnd.choice(live)
         try:
             next(actors[a])
         except StopIteration:
             live.remove(a)
-    assert done_check == [True]
+    assert len(done_check) == 1
+    return done_check[0]
 
 
 @pytest.mark.parametrize("seed", range(200))
 def test_release_elision_publishes_every_item(seed):
-    _elision(seed, n_items=1 + seed % 16, n_xcd=1 + seed % 8)
+    assert _elision(seed, n_items=1 + seed % 16, n_xcd=1 + seed % 8)
+
+
+def test_release_elision_needs_the_store_wait():
+    """Without the per-wave store wait a store can reach the L2 after the last item's write-back:
+    the model must find such an interleaving (it checks that the model can fail at all)."""
+    fails = sum(not _elision(seed, n_items=2 + seed % 15, n_xcd=1 + seed % 4, wait=False) for seed in range(300))
+    assert fails > 0

@@ -186,3 +186,122 @@ def test_graph_write_deps_matches_oracle(ctx, oracle, tmp_path):
     lines = ours.read_text().splitlines()
     assert len(lines) == len(set(lines)) == len(out_idx)
     assert set(lines) == {line for line in ref.read_text().splitlines() if line}
+
+
+def host_threads():
+    """Worker threads for the oracle: the host cores this process may use, capped by OMP_NUM_THREADS when set
+    (the share of the machine the job was given)."""
+    import os
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(env))) if env.isdigit() and int(env) > 0 else n
+
+
+def test_c2_fp64_n16384_matches_oracle(ctx, oracle):
+    # config 2: FP64 tiled Cholesky through the device task DAG at n = 16384, tile 512, A = B B^T/n + I from
+    # seed 2021; L against the oracle's Taskflow + sequential-BLAS factor (threads = host cores) within 1e-12
+    # relative, and the device's own ||L L^T - A||_F / ||A||_F
+    n, b = 16384, 512
+    A = oracle.spd_bbt(n, 2021)
+    Lref, st = oracle.cholesky(A, b, n_threads=host_threads(), prio_kind=2)
+    assert st["info"] == 0
+    M = tt.TiledMatrix(ctx, [b] * (n // b), "f64")
+    M.set(A)
+    W = tt.TiledMatrix(ctx, [b] * (n // b), "f64")
+    W.copy_from(M)
+    info = ctx.potrf_tiled(W, tt.TTAD_SCHED_PERSISTENT, 2, "f64")
+    assert info == 0
+    res = ctx.factor_residual(M, W)
+    L = W.get()
+    W.close()
+    M.close()
+    err = rel(L, Lref)
+    print(f"config 2: n={n} b={b} rel(L, L_oracle)={err:.3e} ||LL^T-A||/||A||={res:.3e} "
+          f"oracle {st['total']:.2f} s on {host_threads()} threads", flush=True)
+    assert err < 1e-12
+    assert res < 1e-14
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16x3", "bf16"])
+def test_factor_residual_matches_host(ctx, oracle, prec):
+    # a low-precision factor's residual (~1e-7 .. 1e-3) is far above the FP64 rounding of either computation,
+    # so the device and the host must agree closely; ragged last tile
+    sizes = [256, 256, 256, 192]
+    n = sum(sizes)
+    A = oracle.spd_bbt(n, 41)
+    M = tt.TiledMatrix(ctx, sizes, "f64")
+    M.set(A)
+    W = tt.TiledMatrix(ctx, sizes, "f32")
+    W.copy_from(M)
+    assert ctx.potrf_tiled(W, tt.TTAD_SCHED_PERSISTENT, 2, prec) == 0
+    dev = ctx.factor_residual(M, W)
+    host = oracle.factor_residual(A, W.get())
+    assert host > 1e-9
+    assert abs(dev - host) <= 1e-6 * host, (dev, host)
+    W.close()
+    M.close()
+
+
+def test_factor_residual_planted_perturbation(ctx, oracle):
+    # L' = L + eps e_p e_q^T (p >= q) gives L'L'^T - A = eps (c e_p^T + e_p c^T + eps e_p e_p^T) + (LL^T - A),
+    # c = L e_q, whose Frobenius norm has the closed form eps sqrt(2 sum_{a != p} c_a^2 + (2 c_p + eps)^2)
+    n, b = 640, 128
+    A = oracle.spd_bbt(n, 43)
+    M = tt.TiledMatrix(ctx, [b] * 5, "f64")
+    M.set(A)
+    W = tt.TiledMatrix(ctx, [b] * 5, "f64")
+    W.copy_from(M)
+    assert ctx.potrf_tiled(W, tt.TTAD_SCHED_PERSISTENT, 2, "f64") == 0
+    base = ctx.factor_residual(M, W)
+    assert base < 1e-14
+    L = W.get()
+    eps, p, q = 1e-5, 517, 200
+    c = L[:, q].copy()
+    Lp = L.copy()
+    Lp[p, q] += eps
+    W.set(Lp)
+    got = ctx.factor_residual(M, W)
+    norm_e = eps * np.sqrt(2.0 * (np.sum(c ** 2) - c[p] ** 2) + (2.0 * c[p] + eps) ** 2)
+    want = norm_e / np.linalg.norm(A)
+    assert abs(got - want) <= 1e-6 * want + 2 * base, (got, want)
+    W.close()
+    M.close()
+
+
+def test_graph_trace_readable_by_reference_tools(ctx, oracle, tmp_path):
+    # a traced device run, read the way tools/ttor_logging.py reads Logger files: one '>run>' row per task whose
+    # name joins the DepsLogger edges, a '>deps>' row per task, times in seconds since the epoch, and every edge
+    # respected (a successor starts after its predecessor's last item ended)
+    import time
+
+    import trace_util
+    nb, b = 6, 128
+    tasks, indeg, prio, out_ptr, out_idx = tt.cholesky_dag(nb, 2)
+    A = oracle.spd_bbt(nb * b, 12)
+    M = tt.TiledMatrix(ctx, [b] * nb, "f64")
+    M.set(A)
+    G = tt.Graph(ctx, tasks, indeg, out_ptr, out_idx, prio)
+    G.bind(0, M)
+    G.seed([tt.task_name(t) for t in tasks].index("POTRF_0"))
+    t_before = time.time()
+    G.launch(trace=True)
+    G.wait(60.0)
+    t_after = time.time()
+    log, deps = tmp_path / "trace.csv", tmp_path / "deps.csv"
+    G.write_trace(str(log), "Wk_")
+    G.write_deps(str(deps))
+    G.close()
+    Lref, _ = oracle.cholesky(A, b, n_threads=2)
+    assert rel(M.get(), Lref) < 1e-12
+    M.close()
+    nodes = trace_util.dag_nodes(log).set_index("task")
+    edges = trace_util.dag_edges(deps)
+    assert nodes.index.is_unique and len(nodes) == len(tasks)
+    assert set(nodes.index) == set(edges["start"]) | set(edges["end"])
+    assert (nodes["time_end"] >= nodes["time_start"]).all()
+    assert nodes["time_start"].min() >= t_before - 1.0 and nodes["time_end"].max() <= t_after + 1.0
+    for u, v in zip(edges["start"], edges["end"]):
+        assert nodes.loc[v, "time_start"] >= nodes.loc[u, "time_end"], (u, v)
+    rows = trace_util.trace_rows(log)
+    assert (rows["what"] == "deps").sum() == len(tasks)
+    assert all(w.startswith("Wk_") for w in rows["who"])

@@ -122,6 +122,51 @@ def test_fp64_requests(ctx, problems):
         srv.close()
 
 
+def test_fallback_forced_stall(ctx, problems):
+    # BF16 needs several refinement steps; with max_iter = 1 the low-precision attempt stops short,
+    # and the request is re-run with an FP64 factor (one step suffices there) before it is reported
+    srv = tt.Server(ctx, hbm_budget=1 << 30, max_tasks=1 << 16, max_requests=16)
+    try:
+        A, M, B = problems[4]
+        rid, _ = srv.submit(M, B, prec="bf16", max_iter=1)
+        r = srv.wait_all([rid])[rid]
+        assert r["status"] == 0 and r["fell_back"] == 1, r
+        assert r["iterations"] == 2, r
+        assert eta(A, r["X"], B) <= 1e-12
+        # the same request in FP64 from the start gives the same solution bit for bit
+        rid2, _ = srv.submit(M, B, prec="f64", max_iter=1)
+        r2 = srv.wait_all([rid2])[rid2]
+        assert r2["status"] == 0 and r2["fell_back"] == 0
+        assert np.array_equal(r["X"], r2["X"])
+    finally:
+        srv.close()
+
+
+def test_fallback_ill_conditioned_and_not_spd(ctx, oracle):
+    # kappa ~ 1e10 (A = D (B B^T/n + delta I) D): the bf16x3 attempt cannot refine to 1e-12 in 10 steps,
+    # the FP64 one does. An indefinite matrix fails both attempts: NOT_SPD, reported once, fell_back = 1.
+    n = 768
+    Aill = oracle.spd_scaled(n, 31, 32, delta=1e-6, dmax=1e2)
+    Abad = _spd(n, 33)
+    Abad[np.diag_indices(n)] -= 3.0
+    Mill, Mbad = _mat(ctx, Aill), _mat(ctx, Abad)
+    B = np.asfortranarray(np.random.default_rng(34).standard_normal((n, 3)))
+    srv = tt.Server(ctx, hbm_budget=1 << 30, max_tasks=1 << 16, max_requests=16)
+    try:
+        ids = [srv.submit(Mill, B, prec="bf16x3", max_iter=10)[0], srv.submit(Mbad, B, prec="bf16x3", max_iter=10)[0]]
+        res = srv.wait_all(ids)
+        ri, rb = res[ids[0]], res[ids[1]]
+        assert ri["status"] == 0 and ri["fell_back"] == 1, ri
+        assert eta(Aill, ri["X"], B) <= 2e-12
+        assert rb["status"] == tt.TTAD_ERR_NOT_SPD and rb["fell_back"] == 1, rb
+        L = np.asfortranarray(Abad.copy())
+        assert rb["info"] == oracle.dpotrf(L)
+    finally:
+        srv.close()
+        Mill.close()
+        Mbad.close()
+
+
 def test_context_close_with_live_objects():
     # closing a context first closes the server and matrices still open on it, so
     # that dropping them afterwards never touches the freed context

@@ -86,6 +86,43 @@ def test_mixed_ir_reaches_1e12(ctx, oracle, prec):
     assert fwd < 10 * kappa * max(st["backward_error"], 1e-16) + 1e-13, (fwd, kappa, st)
 
 
+@pytest.mark.parametrize("prec", ["bf16x3", "f32", "bf16"])
+def test_mixed_tile512_16rhs(ctx, oracle, prec):
+    # the headline's shape at a size the host checks quickly: tile 512 (FP32 tiles of 4 x 4 bf16x3 / FP32
+    # items, 8 row-block POTRF items in FP64 arithmetic), 16 right-hand sides, A = D (B B^T/n + 1e-2 I) D
+    n, b, nrhs = 4096, 512, 16
+    A = oracle.spd_scaled(n, 2022, 2023)
+    M = _mat(ctx, A, b)
+    B = np.asfortranarray(np.random.default_rng(2024).standard_normal((n, nrhs)))
+    dB, dX = _dev_cols(B), _dev_cols(np.zeros((n, nrhs)))
+    st = ctx.posv_mixed(M, prec, nrhs, dB, dX, tol=1e-12, max_iter=30)
+    assert st["rc"] == 0 and st["fell_back"] == 0, st
+    X = from_dev(dX)
+    assert st["backward_error"] <= 1e-12 and backward_error(A, X, B) <= 2e-12, st
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "bf16"])
+def test_mixed_refined_to_floor_matches_lapack(ctx, oracle, prec):
+    # refined until the backward error stops halving (tol below what FP64 reaches), the mixed solution
+    # agrees with LAPACK's FP64 solve to what the condition number allows: both are within ~kappa u of x
+    n, b, nrhs = 2048, 256, 4
+    A = oracle.spd_scaled(n, 2022, 2023)
+    M = _mat(ctx, A, b)
+    B = np.asfortranarray(np.random.default_rng(2024).standard_normal((n, nrhs)))
+    dB, dX = _dev_cols(B), _dev_cols(np.zeros((n, nrhs)))
+    W = tt.TiledMatrix(ctx, M.block_sizes, "f32")
+    W.copy_from(M)
+    assert ctx.potrf_tiled(W, tt.TTAD_SCHED_PERSISTENT, 4, prec) == 0
+    st = ctx.solve_ir(M, W, nrhs, dB, dX, tol=1e-30, max_iter=40)
+    assert st["rc"] == tt.TTAD_ERR_IR_STALLED and st["iterations"] < 40, st  # stopped by the stall rule
+    X = from_dev(dX)
+    assert backward_error(A, X, B) <= 1e-15, st
+    Xref = np.linalg.solve(A, B)
+    kappa = np.linalg.cond(A)
+    fwd = np.linalg.norm(X - Xref) / np.linalg.norm(Xref)
+    assert fwd <= 10 * kappa * np.finfo(np.float64).eps, (fwd, kappa)
+
+
 def test_fallback_on_breakdown(ctx, oracle):
     # kappa ~ 1e12 well beyond what an FP32 factor refines: expect the FP64 fallback
     n, b, nrhs = 1024, 256, 4
@@ -96,8 +133,27 @@ def test_fallback_on_breakdown(ctx, oracle):
     st = ctx.posv_mixed(M, "bf16", nrhs, dB, dX, tol=1e-12, max_iter=10)
     assert st["fell_back"] == 1
     assert st["prec_used"] == tt.TTAD_F64
-    if st["rc"] == 0:
-        assert backward_error(A, from_dev(dX), B) <= 2e-12
+    # the FP64 factor of a kappa ~ 1e12 matrix is backward stable: refinement must reach the target
+    assert st["rc"] == 0, st
+    assert backward_error(A, from_dev(dX), B) <= 2e-12
+
+
+def test_not_spd_input_falls_back_then_fails(ctx, oracle):
+    # an indefinite symmetric matrix (eigenvalues of B B^T/n + I - 3 I in [-2, 2]): the low-precision
+    # factor breaks down, the FP64 one too; the caller gets NOT_SPD with the pivot column
+    n, b, nrhs = 512, 256, 2
+    A = oracle.spd_bbt(n, 9)
+    A[np.diag_indices(n)] -= 3.0
+    M = _mat(ctx, A, b)
+    B = np.asfortranarray(np.random.default_rng(2).standard_normal((n, nrhs)))
+    dB, dX = _dev_cols(B), _dev_cols(np.zeros((n, nrhs)))
+    st = ctx.posv_mixed(M, "bf16x3", nrhs, dB, dX, tol=1e-12, max_iter=10)
+    assert st["rc"] == tt.TTAD_ERR_NOT_SPD, st
+    assert st["fell_back"] == 1 and st["prec_used"] == tt.TTAD_F64
+    assert 1 <= st["info"] <= n
+    # LAPACK agrees that the leading minor of that order is not positive definite
+    L = np.asfortranarray(A.copy())
+    assert oracle.dpotrf(L) == st["info"]
 
 
 def test_factor_reused_for_new_rhs(ctx, oracle):

@@ -209,3 +209,21 @@ def test_block_sizes(oracle, b, nb, ub):
     exp.append(n - acc)
     assert got.tolist() == exp
     assert int(got.sum()) == n
+
+
+def test_oracle_trace_joins_deps_like_the_reference_tools(oracle, tmp_path):
+    # the oracle's Logger / DepsLogger files (restating util.cpp:103-127) read the way tools/ttor_logging.py reads
+    # them: every task of the DAG has one '>run>' row whose name joins the DepsLogger edges, and '>deps>' rows exist
+    import trace_util
+    nb = 5
+    A = oracle.spd_bbt(nb * 16, 3)
+    log, deps = tmp_path / "log.csv", tmp_path / "deps.csv"
+    oracle.cholesky(A, 16, n_threads=3, prio_kind=2, log_path=str(log), deps_path=str(deps))
+    nodes = trace_util.dag_nodes(log)
+    edges = trace_util.dag_edges(deps)
+    names = set(edges["start"]) | set(edges["end"])
+    assert nodes["task"].is_unique
+    assert set(nodes["task"]) == names
+    assert set(nodes["kind"]) == {"POTRF", "TRSM", "GEMM"}
+    rows = trace_util.trace_rows(log)
+    assert set(rows["what"]) >= {"run", "deps"}
