@@ -196,19 +196,30 @@ def _host_eta(A, X, B):
     return float(eta.max())
 
 
-def _request_matrices(ctx, sizes, seeds, deltas=None, dexps=None, tile=256):
-    """One FP64 tiled matrix per request: A = D (B B^T/n + delta I) D, B iid N(0,1) from the request's seed,
-    D_ii = 10^(dexp U_i) (D = I when dexp = 0). Returns [(A dense on the device, TiledMatrix)]."""
+def _request_matrices(ctx, sizes, seeds, kappa_inner=None, dexps=None, tile=256):
+    """One FP64 tiled matrix per request. kappa_inner None: A = B B^T/n + I with B n x n iid N(0,1) from the
+    request's seed. Else A = D (B B^T/n + delta I) D with B n x n/2 (so B B^T/n has n/2 zero eigenvalues and
+    cond(B B^T/n + delta I) = (lambda_max + delta) / delta exactly), delta = lambda_max / (kappa_inner - 1)
+    (lambda_max by power iteration) and D_ii = 10^(dexp U_i). Returns [(A dense on the device, TiledMatrix)]."""
     import torch
     import tasktorrent_amd as tt
     out = []
     for q, n in enumerate(sizes):
         g = torch.Generator(device="cuda")
         g.manual_seed(int(seeds[q]))
-        B = torch.randn((n, n), dtype=torch.float64, device="cuda", generator=g)
+        m = n if kappa_inner is None else max(1, n // 2)
+        B = torch.randn((n, m), dtype=torch.float64, device="cuda", generator=g)
         A = B @ B.T / n
         del B
-        A.diagonal().add_(1.0 if deltas is None else float(deltas[q]))
+        if kappa_inner is None:
+            A.diagonal().add_(1.0)
+        else:
+            v = torch.randn((n, 1), dtype=torch.float64, device="cuda", generator=g)
+            for _ in range(30):
+                v = A @ v
+                v /= torch.linalg.vector_norm(v)
+            lmax = float((v.T @ (A @ v)).item())
+            A.diagonal().add_(lmax / (float(kappa_inner[q]) - 1.0))
         if dexps is not None and dexps[q] > 0:
             d = torch.pow(10.0, float(dexps[q]) * torch.rand((n,), dtype=torch.float64, device="cuda", generator=g))
             A *= d[:, None]
@@ -275,11 +286,12 @@ def serving_leg(ctx, rates, n_req: int, seed: int = 7, prec: str = "bf16x3", bud
 
 
 def stress_leg(ctx, rate: float, n_req: int, seed: int = 11, prec: str = "bf16x3", budget: int = 200 << 30):
-    """Config 5: heavy-tailed sizes n = 256*round(min(16384, Pareto(1.5, x_m = 256))/256), kappa log-uniform in
-    [1e2, 1e10] through A = D (B B^T/n + delta I) D (half of log10 kappa from delta = 4/(10^(L/2) - 1), half from
-    D_ii = 10^(L/4 U_i)), 1-8 RHS, admission by HBM footprint under a 200 GB arena, refinement capped at 10 steps
-    and then the automatic FP64 refactorization. Reports the fallback rate, the backward errors (the server's and
-    an FP64 recomputation of every request's) and the latency percentiles."""
+    """Config 5: heavy-tailed sizes n = 256*round(min(16384, Pareto(1.5, x_m = 256))/256), nominal kappa
+    10^L with L uniform in [2, 10] through A = D (B B^T/n + delta I) D: half of L from delta (cond of the inner
+    matrix exactly 10^(L/2), B n x n/2), half from D_ii = 10^(L/4 U_i). 1-8 RHS, admission by HBM footprint under
+    a 200 GB arena, refinement capped at 10 steps and then the automatic FP64 refactorization. Reports the
+    fallback rate, the backward errors (the server's and an FP64 recomputation of every request's), the
+    measured condition numbers of the small requests and the latency percentiles."""
     import numpy as np
     import tasktorrent_amd as tt
     rng = np.random.default_rng(seed)
@@ -290,9 +302,12 @@ def stress_leg(ctx, rate: float, n_req: int, seed: int = 11, prec: str = "bf16x3
         sizes.append(max(256, n))
         logk.append(float(rng.uniform(2.0, 10.0)))
         reqs.append((q, int(rng.integers(1, 9)), -1))
-    deltas = [4.0 / (10.0 ** (L / 2) - 1.0) for L in logk]
+    kin = [10.0 ** (L / 2) for L in logk]
     dexps = [L / 4.0 for L in logk]
-    mats = _request_matrices(ctx, sizes, [30_000 + q for q in range(n_req)], deltas, dexps)
+    mats = _request_matrices(ctx, sizes, [30_000 + q for q in range(n_req)], kin, dexps)
+    # the condition numbers the construction reaches, measured (2-norm) on the requests with n <= 1024
+    import torch
+    meas = {q: float(torch.linalg.cond(mats[q][0]).item()) for q in range(n_req) if sizes[q] <= 1024}
     srv = tt.Server(ctx, hbm_budget=budget, max_tasks=1 << 22, max_requests=1024)
     try:
         recs, rhs, wall = _submit_plan(ctx, srv, reqs, mats, rate, np.random.default_rng(seed + 1), prec,
@@ -309,6 +324,11 @@ def stress_leg(ctx, rate: float, n_req: int, seed: int = 11, prec: str = "bf16x3
            "kappa_log10_range": [round(min(logk), 2), round(max(logk), 2)],
            "fallback_rate": round(float(fb.mean()), 4), "ok": int(ok.sum()),
            "fallback_kappa_log10_min": round(float(min([logk[q] for q in range(n_req) if fb[q]], default=0)), 2),
+           "no_fallback_kappa_log10_max": round(float(max([logk[q] for q in range(n_req) if not fb[q]], default=0)), 2),
+           "measured_kappa_log10_n_le_1024": [round(float(np.log10(min(meas.values()))), 2),
+                                              round(float(np.log10(max(meas.values()))), 2)] if meas else None,
+           "fallback_rate_measured_kappa_gt_1e8": round(float(np.mean([fb[q] for q in meas if meas[q] > 1e8])), 3)
+           if any(v > 1e8 for v in meas.values()) else None,
            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
            "solves_per_s": round(n_req / wall, 2), "wall_s": round(wall, 3),
            "max_backward_error": float(max(r["eta"] for r in recs)),
@@ -426,7 +446,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cpu-c3", action="store_true", help="CPU baseline at configs 1 and 2 only")
     ap.add_argument("--fp64-only", action="store_true")
-    ap.add_argument("--serve-rates", default="100,200,400,800", help="Poisson request rates of the serving leg (1/s)")
+    ap.add_argument("--serve-rates", default="200,800,1600,3200", help="Poisson request rates of the serving leg (1/s)")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--stress-rate", type=float, default=50.0, help="request rate of the config-5 stress leg (1/s)")

@@ -133,6 +133,7 @@ int ttad_init(int device, ttad_context** out) {
     hipError_t e = hipStreamCreate(&c->stream);
     if (e == hipSuccess) e = kernels_init();
     if (e == hipSuccess) e = factor_check_init();
+    if (e == hipSuccess) e = solve_init();
     if (e == hipSuccess) e = c->scratch_info.alloc(sizeof(int));
     if (e != hipSuccess) {
         delete c;

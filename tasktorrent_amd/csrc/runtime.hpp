@@ -18,6 +18,7 @@ namespace ttad {
 
 int set_error(int code, const char* fmt, ...);
 hipError_t factor_check_init();
+hipError_t solve_init();
 // ||L L^T - A||_F / ||A||_F (factor_check.hip); synchronizes s.
 int factor_residual(ttad_context* ctx, ttad_matrix* A, ttad_matrix* L, double* rel, hipStream_t s);
 
@@ -74,7 +75,7 @@ struct ttad_context {
     bool ev_valid = false;
     double kernel_s = -1.0;  // event time of the last persistent factorization kernel (< 0: stream schedule)
     // solve-phase workspace (grown on demand)
-    ttad::DevBuf sw_Y, sw_D, sw_R, sw_flags, sw_ticket, sw_err, sw_norms, sw_part;
+    ttad::DevBuf sw_Y, sw_D, sw_P, sw_R, sw_flags, sw_ticket, sw_err, sw_norms, sw_part;
     // options (ttad_set_option)
     bool report_factor_residual = false;
     // working copies of FP64 matrices for ttad_posv_mixed, keyed by (matrix id, storage)
@@ -93,7 +94,10 @@ struct ttad_matrix {
     ttad::DevMat host_desc{};
     // 64-row block table of the solve phase (built on first use)
     int sweep_P = 0;
-    ttad::DevBuf tab_blk_tile, tab_blk_off;
+    ttad::DevBuf tab_blk_tile, tab_blk_off, tab_tfirst;
+    // inverses of the diagonal tiles of a factor (tile-blocked sweeps), formed once per solve call
+    ttad::DevBuf tinv, tab_tinv;
+    bool tinv_fresh = false;
     double anorm = 0.0;  // ||A||_inf, valid until the next ttad_matrix_set/copy into this matrix
     bool anorm_valid = false;
 };

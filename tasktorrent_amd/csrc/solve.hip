@@ -8,6 +8,7 @@
 
 #include "runtime.hpp"
 #include "solve_ops.hpp"
+#include "tile_sweep.hpp"
 
 namespace ttad {
 // Integer tuning override from the environment, clamped to [lo, hi].
@@ -32,19 +33,23 @@ static int mixed_prio_kind() {  // TTAD_MIXED_PRIO overrides it (tuning)
 // Per-matrix 64-row block table (built on first use).
 static int ensure_sweep_tab(ttad_matrix* M) {
     if (M->sweep_P > 0) return TTAD_OK;
-    std::vector<int32_t> bt, bo;
+    std::vector<int32_t> bt, bo, tf;
     for (int i = 0; i < M->nt; i++) {
         if (i < M->nt - 1 && M->bsz[i] % SB != 0)
             return set_error(TTAD_ERR_INVALID, "solve: tile %d has size %d, not a multiple of %d", i, M->bsz[i], SB);
+        tf.push_back((int32_t)bt.size());
         for (int o = 0; o < M->bsz[i]; o += SB) {
             bt.push_back(i);
             bo.push_back(o);
         }
     }
+    tf.push_back((int32_t)bt.size());
     TTAD_HIP(M->tab_blk_tile.alloc(std::max<size_t>(1, bt.size()) * 4));
     TTAD_HIP(M->tab_blk_off.alloc(std::max<size_t>(1, bo.size()) * 4));
+    TTAD_HIP(M->tab_tfirst.alloc(tf.size() * 4));
     TTAD_HIP(hipMemcpy(M->tab_blk_tile.p, bt.data(), bt.size() * 4, hipMemcpyHostToDevice));
     TTAD_HIP(hipMemcpy(M->tab_blk_off.p, bo.data(), bo.size() * 4, hipMemcpyHostToDevice));
+    TTAD_HIP(hipMemcpy(M->tab_tfirst.p, tf.data(), tf.size() * 4, hipMemcpyHostToDevice));
     M->sweep_P = (int)bt.size();
     return TTAD_OK;
 }
@@ -119,16 +124,89 @@ static void launch_sweep(ttad_matrix* L, int forward, const double* R, long ldr,
                            ldd, X, ldx, nrhs, flags, ticket, err, spin);
 }
 
+// Tile-blocked sweeps (tile_sweep.hpp) by default; TTAD_SWEEP=block selects the chain of 64-row blocks
+// (for comparisons).
+static bool tile_sweeps() {
+    static const bool on = [] {
+        const char* v = getenv("TTAD_SWEEP");
+        return !(v && strcmp(v, "block") == 0);
+    }();
+    return on;
+}
+
+// Inverses of the diagonal tiles of the factor L (k_tile_inv), unless formed since the caller cleared
+// L->tinv_fresh (every solve call does, so a factor rewritten in between is never used stale).
+static int ensure_tinv(ttad_matrix* L, hipStream_t s) {
+    if (L->tinv_fresh) return TTAD_OK;
+    const int nt = L->nt;
+    size_t tot = 0;
+    std::vector<size_t> off(nt);
+    int maxnb = 1;
+    for (int t = 0; t < nt; t++) {
+        off[t] = tot;
+        tot += ((size_t)L->bsz[t] * L->bsz[t] + 63) / 64 * 64;
+        maxnb = std::max(maxnb, (L->bsz[t] + SB - 1) / SB);
+    }
+    if (!L->tinv.p || L->tinv.bytes < std::max<size_t>(1, tot) * L->elem) {
+        TTAD_HIP(L->tinv.alloc(std::max<size_t>(1, tot) * L->elem));
+        std::vector<void*> ptr(nt);
+        for (int t = 0; t < nt; t++) ptr[t] = (char*)L->tinv.p + off[t] * L->elem;
+        TTAD_HIP(L->tab_tinv.alloc(nt * sizeof(void*)));
+        TTAD_HIP(hipMemcpy(L->tab_tinv.p, ptr.data(), nt * sizeof(void*), hipMemcpyHostToDevice));
+    }
+    const DevMat* Md = L->desc.as<DevMat>();
+    if (L->storage == P_F64)
+        hipLaunchKernelGGL(k_tile_inv<double>, dim3(nt, maxnb), dim3(NTHREADS), TILE_INV_LDS, s, Md,
+                           L->tab_tinv.as<double*>());
+    else
+        hipLaunchKernelGGL(k_tile_inv<float>, dim3(nt, maxnb), dim3(NTHREADS), TILE_INV_LDS, s, Md,
+                           L->tab_tinv.as<float*>());
+    TTAD_HIP(hipGetLastError());
+    L->tinv_fresh = true;
+    return TTAD_OK;
+}
+
+hipError_t solve_init() {
+    hipError_t e = hipFuncSetAttribute((const void*)k_tile_inv<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       TILE_INV_LDS);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_tile_inv<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                TILE_INV_LDS);
+    return e;
+}
+
+template <typename TL, typename TC>
+static void launch_tsweep(ttad_matrix* L, int forward, const double* R, long ldr, TC* Y, long ldy, TC* D, long ldd,
+                          double* X, long ldx, TC* Rp, long ldp, int nrhs, int32_t* flags, int32_t* ticket,
+                          int32_t* err, int64_t spin, hipStream_t s) {
+    const int P = L->sweep_P;
+    hipLaunchKernelGGL(k_zero_i32, dim3((P + 1 + 255) / 256), dim3(256), 0, s, flags, P + 1);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, ticket, 1);
+    const DevMat* Md = L->desc.as<DevMat>();
+    const TileSweepTab tab{P, L->nt, L->tab_blk_tile.as<int32_t>(), L->tab_blk_off.as<int32_t>(),
+                           L->tab_tfirst.as<int32_t>()};
+    TL* const* tinv = L->tab_tinv.as<TL*>();
+    if (nrhs <= 16)
+        hipLaunchKernelGGL((k_tsweep<TL, TC, 16>), dim3(P), dim3(NTHREADS), 0, s, Md, tab, tinv, forward, R, ldr, Y,
+                           ldy, D, ldd, X, ldx, Rp, ldp, nrhs, flags, ticket, err, spin);
+    else
+        hipLaunchKernelGGL((k_tsweep<TL, TC, 32>), dim3(P), dim3(NTHREADS), 0, s, Md, tab, tinv, forward, R, ldr, Y,
+                           ldy, D, ldd, X, ldx, Rp, ldp, nrhs, flags, ticket, err, spin);
+}
+
 // X += (L L^T)^{-1} R, in chunks of at most 32 right-hand sides; work precision of L.
 static int solve_add(ttad_context* ctx, ttad_matrix* L, int nrhs, const double* R, long ldr, double* X, long ldx,
                      hipStream_t s) {
     int rc = ensure_sweep_tab(L);
     if (rc) return rc;
+    const bool tiled = tile_sweeps();
+    if (tiled && (rc = ensure_tinv(L, s))) return rc;
     const int n = L->n, P = L->sweep_P;
     const size_t es = L->storage == P_F64 ? 8 : 4;
     const int chunk = std::min(nrhs, 32);
     if ((rc = ensure(ctx->sw_Y, (size_t)n * chunk * es))) return rc;
     if ((rc = ensure(ctx->sw_D, (size_t)n * chunk * es))) return rc;
+    if (tiled && (rc = ensure(ctx->sw_P, (size_t)n * chunk * es))) return rc;
     if ((rc = ensure(ctx->sw_flags, (size_t)(P + 1) * 4))) return rc;
     if ((rc = ensure(ctx->sw_ticket, 16))) return rc;
     if ((rc = ensure(ctx->sw_err, 16))) return rc;
@@ -143,13 +221,25 @@ static int solve_add(ttad_context* ctx, ttad_matrix* L, int nrhs, const double* 
         if (L->storage == P_F64) {
             double* Y = ctx->sw_Y.as<double>();
             double* D = ctx->sw_D.as<double>();
-            launch_sweep<double, double>(L, 1, Rc, ldr, Y, n, D, n, Xc, ldx, nc, flags, ticket, err, spin, s);
-            launch_sweep<double, double>(L, 0, Rc, ldr, Y, n, D, n, Xc, ldx, nc, flags, ticket, err, spin, s);
+            if (tiled) {
+                double* Rp = ctx->sw_P.as<double>();
+                launch_tsweep<double, double>(L, 1, Rc, ldr, Y, n, D, n, Xc, ldx, Rp, n, nc, flags, ticket, err, spin, s);
+                launch_tsweep<double, double>(L, 0, Rc, ldr, Y, n, D, n, Xc, ldx, Rp, n, nc, flags, ticket, err, spin, s);
+            } else {
+                launch_sweep<double, double>(L, 1, Rc, ldr, Y, n, D, n, Xc, ldx, nc, flags, ticket, err, spin, s);
+                launch_sweep<double, double>(L, 0, Rc, ldr, Y, n, D, n, Xc, ldx, nc, flags, ticket, err, spin, s);
+            }
         } else {
             float* Y = ctx->sw_Y.as<float>();
             float* D = ctx->sw_D.as<float>();
-            launch_sweep<float, float>(L, 1, Rc, ldr, Y, n, D, n, Xc, ldx, nc, flags, ticket, err, spin, s);
-            launch_sweep<float, float>(L, 0, Rc, ldr, Y, n, D, n, Xc, ldx, nc, flags, ticket, err, spin, s);
+            if (tiled) {
+                float* Rp = ctx->sw_P.as<float>();
+                launch_tsweep<float, float>(L, 1, Rc, ldr, Y, n, D, n, Xc, ldx, Rp, n, nc, flags, ticket, err, spin, s);
+                launch_tsweep<float, float>(L, 0, Rc, ldr, Y, n, D, n, Xc, ldx, Rp, n, nc, flags, ticket, err, spin, s);
+            } else {
+                launch_sweep<float, float>(L, 1, Rc, ldr, Y, n, D, n, Xc, ldx, nc, flags, ticket, err, spin, s);
+                launch_sweep<float, float>(L, 0, Rc, ldr, Y, n, D, n, Xc, ldx, nc, flags, ticket, err, spin, s);
+            }
         }
     }
     TTAD_HIP(hipGetLastError());
@@ -239,6 +329,7 @@ static int check_err_flag(ttad_context* ctx, hipStream_t s) {
 static int refine(ttad_context* ctx, ttad_matrix* A, ttad_matrix* L, int nrhs, const double* B, long ldb, double* X,
                   long ldx, double tol, int max_iter, int* iters, double* eta_out, hipStream_t s) {
     const int n = A->n;
+    L->tinv_fresh = false;  // the factor may have changed since the last solve: its tile inverses are formed again
     int rc = ensure(ctx->sw_R, (size_t)n * nrhs * 8);
     if (rc) return rc;
     if ((rc = ensure(ctx->sw_norms, (size_t)(3 * 32 + 8) * 8))) return rc;
@@ -303,6 +394,7 @@ int ttad_solve(ttad_context* ctx, ttad_matrix* L, int nrhs, const double* B, lon
         return set_error(TTAD_ERR_INVALID, "solve: bad args");
     hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
     hipLaunchKernelGGL(k_zero_cols, dim3(256), dim3(256), 0, s, X, ldx, L->n, nrhs);
+    L->tinv_fresh = false;
     int rc = solve_add(ctx, L, nrhs, B, ldb, X, ldx, s);
     if (rc) return rc;
     return check_err_flag(ctx, s);
