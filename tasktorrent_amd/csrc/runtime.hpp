@@ -95,8 +95,10 @@ struct ttad_matrix {
     // 64-row block table of the solve phase (built on first use)
     int sweep_P = 0;
     ttad::DevBuf tab_blk_tile, tab_blk_off, tab_tfirst;
-    // inverses of the diagonal tiles of a factor (tile-blocked sweeps), formed once per solve call
-    ttad::DevBuf tinv, tab_tinv;
+    // operands of the lookahead sweeps of an FP32 factor (inverses of the diagonal tiles, their transposes,
+    // the lookahead products), formed once per solve call
+    ttad::DevBuf tinv, tab_tinv, tab_pairs;
+    int la_items = 0, la_prods = 0;
     bool tinv_fresh = false;
     double anorm = 0.0;  // ||A||_inf, valid until the next ttad_matrix_set/copy into this matrix
     bool anorm_valid = false;

@@ -124,90 +124,146 @@ static void launch_sweep(ttad_matrix* L, int forward, const double* R, long ldr,
                            ldd, X, ldx, nrhs, flags, ticket, err, spin);
 }
 
-// Tile-blocked sweeps (tile_sweep.hpp) by default; TTAD_SWEEP=block selects the chain of 64-row blocks
-// (for comparisons).
-static bool tile_sweeps() {
+// Lookahead sweeps (tile_sweep.hpp) for factors in FP32 tiles of at most 512 rows, 16 right-hand sides
+// at a time; the chain of 64-row blocks otherwise, and always with TTAD_SWEEP=block (for comparisons).
+static bool lookahead_enabled() {
     static const bool on = [] {
         const char* v = getenv("TTAD_SWEEP");
         return !(v && strcmp(v, "block") == 0);
     }();
     return on;
 }
+static bool lookahead_ok(const ttad_matrix* L) {
+    if (!lookahead_enabled() || L->storage != P_F32) return false;
+    for (int b : L->bsz)
+        if (b > LS_BMAX || b <= 0) return false;
+    return true;
+}
 
-// Inverses of the diagonal tiles of the factor L (k_tile_inv), unless formed since the caller cleared
-// L->tinv_fresh (every solve call does, so a factor rewritten in between is never used stale).
-static int ensure_tinv(ttad_matrix* L, hipStream_t s) {
+// Operands of the lookahead sweeps for the factor L (FP32 tiles): Tinv_t, U_t = Tinv_t^T, M_t, KT_t, formed
+// from L's tiles and POTRF's diagonal-block inverses unless L->tinv_fresh (every solve call clears it, so a
+// factor rewritten in between is never used stale).
+static int ensure_lookahead(ttad_matrix* L, hipStream_t s) {
     if (L->tinv_fresh) return TTAD_OK;
     const int nt = L->nt;
+    auto up = [](size_t x) { return (x + 63) / 64 * 64; };  // 256-byte aligned sections
+    std::vector<size_t> o_inv(nt), o_u(nt), o_lt(nt, 0), o_m(nt, 0), o_k(nt, 0), o_tf(nt), o_uf(nt);
     size_t tot = 0;
-    std::vector<size_t> off(nt);
     int maxnb = 1;
     for (int t = 0; t < nt; t++) {
-        off[t] = tot;
-        tot += ((size_t)L->bsz[t] * L->bsz[t] + 63) / 64 * 64;
+        const size_t bt = L->bsz[t];
         maxnb = std::max(maxnb, (L->bsz[t] + SB - 1) / SB);
+        o_inv[t] = tot;
+        tot += up(bt * bt);
+        o_u[t] = tot;
+        tot += up(bt * bt);
+        o_tf[t] = tot;  // fragment-order copies: 512 x 512, zero-padded
+        tot += (size_t)LS_BMAX * LS_BMAX;
+        o_uf[t] = tot;
+        tot += (size_t)LS_BMAX * LS_BMAX;
+        if (t + 1 < nt) {
+            o_lt[t] = tot;
+            tot += up(bt * L->bsz[t + 1]);
+            o_k[t] = tot;
+            tot += up(bt * L->bsz[t + 1]);
+        }
+        if (t >= 1) {
+            o_m[t] = tot;
+            tot += up(bt * L->bsz[t - 1]);
+        }
     }
-    if (!L->tinv.p || L->tinv.bytes < std::max<size_t>(1, tot) * L->elem) {
-        TTAD_HIP(L->tinv.alloc(std::max<size_t>(1, tot) * L->elem));
-        std::vector<void*> ptr(nt);
-        for (int t = 0; t < nt; t++) ptr[t] = (char*)L->tinv.p + off[t] * L->elem;
-        TTAD_HIP(L->tab_tinv.alloc(nt * sizeof(void*)));
-        TTAD_HIP(hipMemcpy(L->tab_tinv.p, ptr.data(), nt * sizeof(void*), hipMemcpyHostToDevice));
+    if (!L->tinv.p || L->tinv.bytes < std::max<size_t>(1, tot) * 4) {
+        TTAD_HIP(L->tinv.alloc(std::max<size_t>(1, tot) * 4));
+        TTAD_HIP(hipMemset(L->tinv.p, 0, L->tinv.bytes));  // the padding of the fragment-order copies stays zero
+        float* base = L->tinv.as<float>();
+        std::vector<float*> ptr((size_t)7 * nt, nullptr);
+        for (int t = 0; t < nt; t++) {
+            ptr[t] = base + o_inv[t];
+            ptr[nt + t] = base + o_u[t];
+            ptr[2 * nt + t] = t >= 1 ? base + o_m[t] : nullptr;
+            ptr[3 * nt + t] = t + 1 < nt ? base + o_k[t] : nullptr;
+            ptr[4 * nt + t] = t + 1 < nt ? base + o_lt[t] : nullptr;
+            ptr[5 * nt + t] = base + o_tf[t];
+            ptr[6 * nt + t] = base + o_uf[t];
+        }
+        // products of the lookahead: M_t = L_{t,t-1} U_{t-1}^T and KT_t = LT_t Tinv_{t+1}^T, as A B^T
+        std::vector<PairGemm> pg;
+        int items = 0;
+        auto tile = [&](int i, int j) { return (const float*)((char*)L->tiles.p + L->tile_off[i + (size_t)j * nt] * 4); };
+        for (int t = 1; t < nt; t++) {
+            const int m = L->bsz[t], k = L->bsz[t - 1];
+            pg.push_back(PairGemm{tile(t, t - 1), base + o_u[t - 1], base + o_m[t], m, k, m, m, k, k, items});
+            items += ((m + GEMM_BM - 1) / GEMM_BM) * ((k + GEMM_BM - 1) / GEMM_BM);
+        }
+        for (int t = 0; t + 1 < nt; t++) {
+            const int m = L->bsz[t], k = L->bsz[t + 1];
+            pg.push_back(PairGemm{base + o_lt[t], base + o_inv[t + 1], base + o_k[t], m, k, m, m, k, k, items});
+            items += ((m + GEMM_BM - 1) / GEMM_BM) * ((k + GEMM_BM - 1) / GEMM_BM);
+        }
+        L->la_items = items;
+        L->la_prods = (int)pg.size();
+        TTAD_HIP(L->tab_tinv.alloc(ptr.size() * sizeof(float*)));
+        TTAD_HIP(hipMemcpy(L->tab_tinv.p, ptr.data(), ptr.size() * sizeof(float*), hipMemcpyHostToDevice));
+        TTAD_HIP(L->tab_pairs.alloc(std::max<size_t>(1, pg.size()) * sizeof(PairGemm)));
+        if (!pg.empty())
+            TTAD_HIP(hipMemcpy(L->tab_pairs.p, pg.data(), pg.size() * sizeof(PairGemm), hipMemcpyHostToDevice));
     }
+    float* const* tab = L->tab_tinv.as<float*>();
     const DevMat* Md = L->desc.as<DevMat>();
-    if (L->storage == P_F64)
-        hipLaunchKernelGGL(k_tile_inv<double>, dim3(nt, maxnb), dim3(NTHREADS), TILE_INV_LDS, s, Md,
-                           L->tab_tinv.as<double*>());
-    else
-        hipLaunchKernelGGL(k_tile_inv<float>, dim3(nt, maxnb), dim3(NTHREADS), TILE_INV_LDS, s, Md,
-                           L->tab_tinv.as<float*>());
+    hipLaunchKernelGGL(k_tile_inv<float>, dim3(nt, maxnb), dim3(NTHREADS), TILE_INV_LDS, s, Md, tab, tab + nt,
+                       tab + 5 * nt, tab + 6 * nt);
+    if (nt > 1) {
+        hipLaunchKernelGGL(k_sub_transpose<float>, dim3(nt - 1, maxnb * maxnb), dim3(NTHREADS), 0, s, Md, tab + 4 * nt);
+        constexpr int lds_pg = gemm_item_lds<float, P_F32>();
+        hipLaunchKernelGGL(k_pair_gemm, dim3(L->la_items), dim3(NTHREADS), lds_pg, s, L->tab_pairs.as<PairGemm>(),
+                           L->la_prods);
+    }
     TTAD_HIP(hipGetLastError());
     L->tinv_fresh = true;
     return TTAD_OK;
 }
 
 hipError_t solve_init() {
-    hipError_t e = hipFuncSetAttribute((const void*)k_tile_inv<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    hipError_t e = hipFuncSetAttribute((const void*)k_tile_inv<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        TILE_INV_LDS);
     if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void*)k_tile_inv<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                TILE_INV_LDS);
+        e = hipFuncSetAttribute((const void*)k_lsweep, hipFuncAttributeMaxDynamicSharedMemorySize, LSWEEP_LDS);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_pair_gemm, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (gemm_item_lds<float, P_F32>()));
     return e;
 }
 
-template <typename TL, typename TC>
-static void launch_tsweep(ttad_matrix* L, int forward, const double* R, long ldr, TC* Y, long ldy, TC* D, long ldd,
-                          double* X, long ldx, TC* Rp, long ldp, int nrhs, int32_t* flags, int32_t* ticket,
-                          int32_t* err, int64_t spin, hipStream_t s) {
-    const int P = L->sweep_P;
-    hipLaunchKernelGGL(k_zero_i32, dim3((P + 1 + 255) / 256), dim3(256), 0, s, flags, P + 1);
+static void launch_lsweep(ttad_matrix* L, int forward, const double* R, long ldr, float* Y, float* D, double* X,
+                          long ldx, float* Rp, int nrhs, int32_t* flags, int32_t* ticket, int32_t* err, int64_t spin,
+                          hipStream_t s) {
+    const int P = L->sweep_P, nt = L->nt;
+    hipLaunchKernelGGL(k_zero_i32, dim3((2 * nt + 255) / 256), dim3(256), 0, s, flags, 2 * nt);
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, ticket, 1);
-    const DevMat* Md = L->desc.as<DevMat>();
-    const TileSweepTab tab{P, L->nt, L->tab_blk_tile.as<int32_t>(), L->tab_blk_off.as<int32_t>(),
-                           L->tab_tfirst.as<int32_t>()};
-    TL* const* tinv = L->tab_tinv.as<TL*>();
-    if (nrhs <= 16)
-        hipLaunchKernelGGL((k_tsweep<TL, TC, 16>), dim3(P), dim3(NTHREADS), 0, s, Md, tab, tinv, forward, R, ldr, Y,
-                           ldy, D, ldd, X, ldx, Rp, ldp, nrhs, flags, ticket, err, spin);
-    else
-        hipLaunchKernelGGL((k_tsweep<TL, TC, 32>), dim3(P), dim3(NTHREADS), 0, s, Md, tab, tinv, forward, R, ldr, Y,
-                           ldy, D, ldd, X, ldx, Rp, ldp, nrhs, flags, ticket, err, spin);
+    float* const* tab = L->tab_tinv.as<float*>();
+    const LSweepTab lt{P, nt, L->tab_blk_tile.as<int32_t>(), L->tab_blk_off.as<int32_t>(), L->tab_tfirst.as<int32_t>(),
+                       tab, tab + nt, tab + 2 * nt, tab + 3 * nt, tab + 5 * nt, tab + 6 * nt};
+    hipLaunchKernelGGL(k_lsweep, dim3(P), dim3(NTHREADS), LSWEEP_LDS, s, L->desc.as<DevMat>(), lt, forward, R, ldr, Y,
+                       D, X, ldx, Rp, nrhs, flags, ticket, err, spin);
 }
 
-// X += (L L^T)^{-1} R, in chunks of at most 32 right-hand sides; work precision of L.
+// X += (L L^T)^{-1} R in the work precision of L: lookahead sweeps (16 right-hand sides at a time) where they
+// apply, else the 64-row block chain (at most 32 at a time).
 static int solve_add(ttad_context* ctx, ttad_matrix* L, int nrhs, const double* R, long ldr, double* X, long ldx,
                      hipStream_t s) {
     int rc = ensure_sweep_tab(L);
     if (rc) return rc;
-    const bool tiled = tile_sweeps();
-    if (tiled && (rc = ensure_tinv(L, s))) return rc;
+    const bool la = lookahead_ok(L);
+    if (la && (rc = ensure_lookahead(L, s))) return rc;
     const int n = L->n, P = L->sweep_P;
     const size_t es = L->storage == P_F64 ? 8 : 4;
-    const int chunk = std::min(nrhs, 32);
-    if ((rc = ensure(ctx->sw_Y, (size_t)n * chunk * es))) return rc;
-    if ((rc = ensure(ctx->sw_D, (size_t)n * chunk * es))) return rc;
-    if (tiled && (rc = ensure(ctx->sw_P, (size_t)n * chunk * es))) return rc;
-    if ((rc = ensure(ctx->sw_flags, (size_t)(P + 1) * 4))) return rc;
+    const int chunk = la ? std::min(nrhs, LS_NR) : std::min(nrhs, 32);
+    // the lookahead sweeps' row vectors are padded by a tile: their loads of the last tile stay inside
+    const size_t rows = la ? (size_t)n + LS_BMAX : (size_t)n, cols = la ? LS_NR : chunk;
+    if ((rc = ensure(ctx->sw_Y, rows * cols * es))) return rc;
+    if ((rc = ensure(ctx->sw_D, rows * cols * es))) return rc;
+    if (la && (rc = ensure(ctx->sw_P, rows * LS_NR * 4))) return rc;
+    if ((rc = ensure(ctx->sw_flags, (size_t)2 * (P + 1) * 4))) return rc;
     if ((rc = ensure(ctx->sw_ticket, 16))) return rc;
     if ((rc = ensure(ctx->sw_err, 16))) return rc;
     const int64_t spin = spin_ticks(ctx);
@@ -218,28 +274,22 @@ static int solve_add(ttad_context* ctx, ttad_matrix* L, int nrhs, const double* 
         const int nc = std::min(chunk, nrhs - c0);
         const double* Rc = R + (long)c0 * ldr;
         double* Xc = X + (long)c0 * ldx;
-        if (L->storage == P_F64) {
+        if (la) {
+            float* Y = ctx->sw_Y.as<float>();
+            float* D = ctx->sw_D.as<float>();
+            float* Rp = ctx->sw_P.as<float>();
+            launch_lsweep(L, 1, Rc, ldr, Y, D, Xc, ldx, Rp, nc, flags, ticket, err, spin, s);
+            launch_lsweep(L, 0, Rc, ldr, Y, D, Xc, ldx, Rp, nc, flags, ticket, err, spin, s);
+        } else if (L->storage == P_F64) {
             double* Y = ctx->sw_Y.as<double>();
             double* D = ctx->sw_D.as<double>();
-            if (tiled) {
-                double* Rp = ctx->sw_P.as<double>();
-                launch_tsweep<double, double>(L, 1, Rc, ldr, Y, n, D, n, Xc, ldx, Rp, n, nc, flags, ticket, err, spin, s);
-                launch_tsweep<double, double>(L, 0, Rc, ldr, Y, n, D, n, Xc, ldx, Rp, n, nc, flags, ticket, err, spin, s);
-            } else {
-                launch_sweep<double, double>(L, 1, Rc, ldr, Y, n, D, n, Xc, ldx, nc, flags, ticket, err, spin, s);
-                launch_sweep<double, double>(L, 0, Rc, ldr, Y, n, D, n, Xc, ldx, nc, flags, ticket, err, spin, s);
-            }
+            launch_sweep<double, double>(L, 1, Rc, ldr, Y, n, D, n, Xc, ldx, nc, flags, ticket, err, spin, s);
+            launch_sweep<double, double>(L, 0, Rc, ldr, Y, n, D, n, Xc, ldx, nc, flags, ticket, err, spin, s);
         } else {
             float* Y = ctx->sw_Y.as<float>();
             float* D = ctx->sw_D.as<float>();
-            if (tiled) {
-                float* Rp = ctx->sw_P.as<float>();
-                launch_tsweep<float, float>(L, 1, Rc, ldr, Y, n, D, n, Xc, ldx, Rp, n, nc, flags, ticket, err, spin, s);
-                launch_tsweep<float, float>(L, 0, Rc, ldr, Y, n, D, n, Xc, ldx, Rp, n, nc, flags, ticket, err, spin, s);
-            } else {
-                launch_sweep<float, float>(L, 1, Rc, ldr, Y, n, D, n, Xc, ldx, nc, flags, ticket, err, spin, s);
-                launch_sweep<float, float>(L, 0, Rc, ldr, Y, n, D, n, Xc, ldx, nc, flags, ticket, err, spin, s);
-            }
+            launch_sweep<float, float>(L, 1, Rc, ldr, Y, n, D, n, Xc, ldx, nc, flags, ticket, err, spin, s);
+            launch_sweep<float, float>(L, 0, Rc, ldr, Y, n, D, n, Xc, ldx, nc, flags, ticket, err, spin, s);
         }
     }
     TTAD_HIP(hipGetLastError());

@@ -1,38 +1,66 @@
-// Triangular sweeps blocked by factor tile, with the inverses of the diagonal tiles.
+// Triangular sweeps with a factor in FP32 tiles, chained over tiles with a one-tile lookahead.
 //
 // The 64-row block sweeps (item_sweep) chain every block on its predecessor: at n = 32768 that is 512
-// dependent steps per sweep, each a counter round trip plus a 64 x 64 product. Here the dependency
-// chain runs over tiles instead. Forward sweep L y = r, for the 64-row block s of tile t (one
-// workgroup each):
-//   1. acc = sum over the blocks q of earlier tiles of L[p, q] y[q], as each earlier tile completes;
-//   2. r'[p] = r[p] - acc is published, and the tile's counter of published blocks is incremented;
-//   3. once every block of tile t has published, y[p] = Tinv_t[s, :] r'_t (Tinv_t = L_tt^{-1}).
-// The backward sweep L^T x = y runs the same way over transposed blocks, from the last tile to the
-// first, with x[p] = (Tinv_t^T)[s, :] r'_t and X += x. So one tile costs two counter round trips
-// instead of one per 64-row block. Tinv_t is formed once per factor (k_tile_inv) from the inverses
-// of the 64 x 64 diagonal blocks that POTRF leaves: X_IJ = -X_II sum_{Q=J}^{I-1} L_IQ X_QJ.
-// Products with an explicit inverse are backward stable for a well-conditioned diagonal tile and
-// lose accuracy like cond(L_tt) otherwise; the refinement that follows every solve absorbs it.
+// dependent steps per sweep, each a counter round trip, a load of the predecessor's block and two small
+// products (about 7 us). Here the chain runs over tiles (64 steps at b = 512), and each step is a single
+// counter round trip followed by one product whose matrix operand is already in LDS.
+//
+// Forward sweep L y = r, with Tinv_t = L_tt^{-1} and r'_t = r_t - sum_{u<t} L_tu y_u, so that y_t = Tinv_t r'_t.
+// The term of the previous tile is written through r'_{t-1}: L_{t,t-1} y_{t-1} = M_t r'_{t-1} with
+// M_t = L_{t,t-1} Tinv_{t-1}, formed once per factor. Workgroup (t, s) owns the 64 rows s of tile t:
+//   A. acc = sum_{u <= t-2} L_tu[s, :] y_u, each tile u as soon as all its blocks of y are final;
+//   B. acc += M_t[s, :] r'_{t-1} once every block of r'_{t-1} is published (M_t[s, :] staged in LDS ahead);
+//   2. r'_t[s] = r_t[s] - acc is published (tile counter of r');
+//   C. once all of r'_t is published, y_t[s] = Tinv_t[s, :] r'_t (Tinv_t[s, :] staged in LDS meanwhile).
+// So the chain r'_{t-1} -> r'_t costs one counter round trip and one 64 x b x 16 product; y_t follows off
+// the chain. The backward sweep L^T x = y runs the same way from the last tile, with U_t = Tinv_t^T,
+// KT_t = (Tinv_{t+1} L_{t+1,t})^T, x_t = U_t r''_t and X += x.
+// Products with explicit inverses are backward stable for well-conditioned diagonal tiles and lose accuracy
+// like cond(L_tt) otherwise; the FP64 refinement that follows every solve with an FP32 factor absorbs it.
+//
+// Right-hand sides: 16 per sweep; Y, D and r' are row-major n x 16 FP32 (one 64-byte row per matrix row),
+// so that the rows of a tile load as whole 256-byte lines. Tiles of at most 512 rows.
 #pragma once
 #include "solve_ops.hpp"
 
 namespace ttad {
 
-struct TileSweepTab {
+constexpr int LS_NR = 16;               // right-hand sides per sweep
+constexpr int LS_BMAX = 512;            // largest tile
+constexpr int LS_KS = LS_BMAX / 4;      // MFMA k-steps of a tile-wide product
+constexpr int LS_LDK = LS_BMAX + 4;     // LDS pitch of a staged 64 x b operand (row-major, k contiguous)
+constexpr int LSWEEP_LDS = SB * LS_LDK * 4;
+constexpr int TILE_INV_LDS = 3 * DB * DP * (int)sizeof(double);
+
+struct LSweepTab {
     int32_t P;                // 64-row blocks
     int32_t nt;               // tiles
     const int32_t* blk_tile;  // [P]
     const int32_t* blk_off;   // [P]
     const int32_t* tfirst;    // [nt + 1]: first block of each tile
+    float* const* tinv;       // [nt] Tinv_t (b_t x b_t, column-major, upper part zero)
+    float* const* uinv;       // [nt] U_t = Tinv_t^T
+    float* const* mfac;       // [nt] M_t = L_{t,t-1} Tinv_{t-1} (b_t x b_{t-1}); t >= 1
+    float* const* kfac;       // [nt] KT_t = (Tinv_{t+1} L_{t+1,t})^T (b_t x b_{t+1}); t <= nt - 2
+    float* const* tfrag;      // [nt] Tinv_t in MFMA A-fragment order (fragment_index), 512 x 512 zero-padded
+    float* const* ufrag;      // [nt] U_t likewise
 };
 
-constexpr int TILE_INV_LDS = 3 * DB * DP * (int)sizeof(double);
+// Position of element (r, c) of a 512 x 512 operand stored in MFMA A-fragment order: for the 64-row block r / 64,
+// wave (r % 64) / 16 and lane 16 (c % 4) + r % 16, the k-steps ks = c / 4 in groups of four (16 bytes).
+__host__ __device__ inline long fragment_index(int r, int c) {
+    const int ks = c >> 2, lane = ((c & 3) << 4) + (r & 15);
+    return ((((long)(r >> 6) * 4 + ((r >> 4) & 3)) * (LS_KS / 4) + (ks >> 2)) * 64 + lane) * 4 + (ks & 3);
+}
 
-// Inverse of the diagonal tile t of a factor: Tinv[t] (b_t x b_t, column-major, ld b_t, lower triangular; the
-// strict upper part is not written). Workgroup (t, J) forms block column J from the top down. Arithmetic in
-// FP64; results stored in the factor's precision and read back for the next blocks of the column.
+// ---------------------------------------------------------------------------
+// Once per factor: Tinv_t and U_t = Tinv_t^T of every diagonal tile (workgroup (t, J): block column J of
+// Tinv_t, top down: X_JJ = inverse of the 64 x 64 diagonal block that POTRF left, X_IJ = -X_II sum_{Q=J}^{I-1}
+// L_IQ X_QJ), with zeros above (Tinv) / below (U) the diagonal blocks. FP64 arithmetic, stored in TL.
+// ---------------------------------------------------------------------------
 template <typename TL>
-__global__ __launch_bounds__(NTHREADS) void k_tile_inv(const DevMat* Mp, TL* const* tinv) {
+__global__ __launch_bounds__(NTHREADS) void k_tile_inv(const DevMat* Mp, TL* const* tinv, TL* const* uinv,
+                                                       TL* const* tfrag, TL* const* ufrag) {
     extern __shared__ __align__(16) char lds[];
     const DevMat& M = *Mp;
     const int t = blockIdx.x, J = blockIdx.y;
@@ -45,14 +73,24 @@ __global__ __launch_bounds__(NTHREADS) void k_tile_inv(const DevMat* Mp, TL* con
     const TTAD_GLOBAL TL* L = gmem((const TL*)M.tiles[t + (long)t * M.nt]);
     const TTAD_GLOBAL double* dv = gmem((const double*)M.dinv[t]);
     TTAD_GLOBAL TL* X = gmem(tinv[t]);
+    TTAD_GLOBAL TL* U = gmem(uinv[t]);
+    TTAD_GLOBAL TL* XF = gmem(tfrag[t]);
+    TTAD_GLOBAL TL* UF = gmem(ufrag[t]);
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     auto rows = [&](int I) { return min(SB, bt - I * SB); };
     const int wJ = rows(J);
+    auto put = [&](int r, int c, double v) {  // Tinv(r, c) and U(c, r), column-major and in fragment order
+        X[r + (long)c * bt] = (TL)v;
+        U[c + (long)r * bt] = (TL)v;
+        XF[fragment_index(r, c)] = (TL)v;
+        UF[fragment_index(c, r)] = (TL)v;
+    };
+    for (int e = tid; e < J * SB * wJ; e += NTHREADS) put(e % (J * SB), J * SB + e / (J * SB), 0.0);
     for (int e = tid; e < DB * DB; e += NTHREADS) {
         const int i = e % DB, j = e / DB;
-        if (i < wJ && j < wJ) X[(J * SB + i) + (long)(J * SB + j) * bt] = (TL)dv[(long)J * DB * DB + i + j * DB];
+        if (i < wJ && j < wJ) put(J * SB + i, J * SB + j, dv[(long)J * DB * DB + i + j * DB]);
     }
-    // each wave's stores reach the L2 before the other waves read them back
+    // each wave's stores reach the L2 before other waves read them back (vmcnt(0) expcnt(7) lgkmcnt(15))
     __builtin_amdgcn_s_waitcnt(0x0F70);
     __syncthreads();
     for (int I = J + 1; I < nbt; I++) {
@@ -95,7 +133,7 @@ __global__ __launch_bounds__(NTHREADS) void k_tile_inv(const DevMat* Mp, TL* con
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = 16 * bi + (lane >> 4) + 4 * r, col = 16 * bj + (lane & 15);
-                if (row < wI && col < wJ) X[(I * SB + row) + (long)(J * SB + col) * bt] = (TL)(-o[r]);
+                if (row < wI && col < wJ) put(I * SB + row, J * SB + col, -o[r]);
             }
         }
         __builtin_amdgcn_s_waitcnt(0x0F70);
@@ -103,189 +141,315 @@ __global__ __launch_bounds__(NTHREADS) void k_tile_inv(const DevMat* Mp, TL* con
     }
 }
 
-// A operand block of Tinv_t for wave w (rows 16w.. of the 64-row block): trans == false: rows of block s,
-// columns of block c; trans == true: A(m, k) = Tinv_t(64c + k, 64s + m). Outside the tile reads as 0.
+// LT_t = L_{t+1,t}^T (b_t x b_{t+1}, column-major, ld b_t) for t = 0 .. nt-2: workgroup (t, 64 x 64 block).
 template <typename TL>
-__device__ __forceinline__ void load_tinv(TL (&a)[KS], const TL* T, int bt, int s, int c, bool trans) {
+__global__ __launch_bounds__(NTHREADS) void k_sub_transpose(const DevMat* Mp, TL* const* lt) {
+    __shared__ TL S[SB][SB + 1];
+    const DevMat& M = *Mp;
+    const int t = blockIdx.x;
+    const int m = M.bsz[t + 1], n = M.bsz[t];  // L_{t+1,t}: m x n, ld m
+    const int bi = blockIdx.y % ((m + SB - 1) / SB), bj = blockIdx.y / ((m + SB - 1) / SB);
+    if (bj * SB >= n) return;
+    const TTAD_GLOBAL TL* L = gmem((const TL*)M.tiles[(t + 1) + (long)t * M.nt]);
+    TTAD_GLOBAL TL* O = gmem(lt[t]);
+    for (int e = threadIdx.x; e < SB * SB; e += NTHREADS) {
+        const int i = e % SB, j = e / SB, r = bi * SB + i, c = bj * SB + j;
+        S[j][i] = (r < m && c < n) ? L[r + (long)c * m] : TL(0);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < SB * SB; e += NTHREADS) {
+        const int i = e % SB, j = e / SB, r = bj * SB + i, c = bi * SB + j;  // O(r, c) = L(c, r)
+        if (r < n && c < m) O[r + (long)c * n] = S[i][j];
+    }
+}
+
+// Batched C_q = A_q B_q^T (beta = 0) over 128 x 128 output items: the M_t and KT_t products.
+struct PairGemm {
+    const float* A;
+    const float* B;
+    float* C;
+    int32_t lda, ldb, ldc, m, n, K;
+    int32_t item0;  // first item of this product
+};
+__global__ __launch_bounds__(NTHREADS, 2) void k_pair_gemm(const PairGemm* prods, int n_prods) {
+    extern __shared__ __align__(16) char lds[];
+    int lo = 0, hi = n_prods - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prods[mid].item0 <= (int)blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const PairGemm g = prods[lo];
+    const int it = blockIdx.x - g.item0, tr = (g.m + GEMM_BM - 1) / GEMM_BM;
+    item_gemm<float, P_F32>(g.A, g.lda, g.B, g.ldb, g.C, g.ldc, g.m, g.n, g.K, (it % tr) * GEMM_BM,
+                            (it / tr) * GEMM_BM, false, 1.0, 0.0, lds);
+}
+
+// ---------------------------------------------------------------------------
+// Lookahead sweep
+// ---------------------------------------------------------------------------
+// A value the whole wave holds alike, read from memory: passed through a scalar register so that branches
+// on it are scalar and loads under them are not exec-masked.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// B fragments of a tile-wide product: rows [g0, g0 + bu) of a row-major n x 16 FP32 array, k = 4 ks + lane/16,
+// n = lane % 16; rows at or beyond bu read as 0. One load per k-step, all in flight together.
+// Loads are unconditional from clamped (in-bounds) addresses, with out-of-range values selected to 0: no
+// branch separates them, so all of them are in flight together.
+// Layout of the sweeps' row vectors (Y, D, r'; n x 16 FP32): the 16-row groups of each tile hold, per MFMA lane,
+// four consecutive k-steps together, so that a lane loads the B fragments of 4 k-steps with one 16-byte load.
+// Element (k, n) of the tile that starts at row g0 (k tile-local, k = 4 ks + kl) lies at
+// 16 g0 + (((ks / 4) * 64 + 16 kl + n) * 4 + ks % 4); tiles start at multiples of 64 rows.
+__device__ __forceinline__ long swz(int g0, int k, int n) {
+    const int ks = k >> 2, kl = k & 3;
+    return (long)g0 * LS_NR + ((((ks >> 2) << 6) + (kl << 4) + n) << 2) + (ks & 3);
+}
+
+// B fragments of a tile-wide product from a swizzled row vector: the tile at row g0 with bu rows; rows at or
+// beyond bu read as 0. 32 loads of 16 bytes per lane, all in flight together.
+__device__ __forceinline__ void load_rows16(float (&b)[LS_KS], const float* Y, int g0, int bu) {
+    const int lane = threadIdx.x & 63;
+    typedef float f4a_t __attribute__((ext_vector_type(4)));
+    const TTAD_GLOBAL f4a_t* p = reinterpret_cast<const TTAD_GLOBAL f4a_t*>(gmem(Y) + (long)g0 * LS_NR) + lane;
+    if (bu == LS_BMAX) {  // whole tile (uniform): no condition at all
+#pragma unroll
+        for (int j = 0; j < LS_KS / 4; j++) {
+            const f4a_t v = p[j * 64];
+#pragma unroll
+            for (int q = 0; q < 4; q++) b[4 * j + q] = v[q];
+        }
+        return;
+    }
+    const int kl = lane >> 4;
+#pragma unroll
+    for (int j = 0; j < LS_KS / 4; j++) {  // the buffer is padded past the last tile: these loads stay in it
+        const f4a_t v = p[j * 64];
+#pragma unroll
+        for (int q = 0; q < 4; q++) b[4 * j + q] = 4 * (4 * j + q) + kl < bu ? v[q] : 0.0f;
+    }
+}
+
+// As[m][k] = A(m, k) for the w rows and bk columns of a column-major operand (ld), zero elsewhere in the
+// 64 x LS_BMAX window. Thread tid loads rows 4 (tid % 16) .. +3 of every 16th column, all loads first, from
+// clamped addresses (values outside selected to 0).
+__device__ __forceinline__ void stage_rows(float* As, const float* A, long ld, int w, int bk) {
+    const int r4 = (threadIdx.x & 15) * 4, c0 = threadIdx.x >> 4;
+    typedef float f4u_t __attribute__((ext_vector_type(4), aligned(4)));
+    f4_t v[LS_BMAX / 16];
+    const TTAD_GLOBAL float* Ag = gmem(A);
+    if (w == SB && bk == LS_BMAX) {  // whole 64 x 512 window (uniform): one 16-byte load per column
+#pragma unroll
+        for (int q = 0; q < LS_BMAX / 16; q++) {
+            const f4u_t x = *reinterpret_cast<const TTAD_GLOBAL f4u_t*>(Ag + r4 + (long)(c0 + 16 * q) * ld);
+            v[q] = f4_t{x[0], x[1], x[2], x[3]};
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < LS_BMAX / 16; q++) {
+            const int c = c0 + 16 * q;
+            const long cc = (long)min(c, bk - 1) * ld;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float x = Ag[min(r4 + i, w - 1) + cc];
+                v[q][i] = (c < bk && r4 + i < w) ? x : 0.0f;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < LS_BMAX / 16; q++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) As[(r4 + i) * LS_LDK + c0 + 16 * q] = v[q][i];
+}
+
+// acc += As(64 x LS_BMAX) * B(LS_BMAX x 16), A from LDS (zero beyond the operand), B fragments in registers
+// (zero beyond the tile); wave w owns rows 16w .. 16w + 15. Four accumulators shorten the chain of dependent
+// MFMAs.
+__device__ __forceinline__ void lds_tile_mma(f4_t& acc, const float* As, const float (&b)[LS_KS]) {
+    const int lane = threadIdx.x & 63;
+    const float* a = As + ((threadIdx.x >> 6) * 16 + (lane & 15)) * LS_LDK + (lane >> 4);
+    f4_t c[4] = {f4_t{0, 0, 0, 0}, f4_t{0, 0, 0, 0}, f4_t{0, 0, 0, 0}, f4_t{0, 0, 0, 0}};
+#pragma unroll
+    for (int ks = 0; ks < LS_KS; ks++) c[ks & 3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[4 * ks], b[ks], c[ks & 3], 0, 0, 0);
+    acc += (c[0] + c[1]) + (c[2] + c[3]);
+}
+
+// A operand of wave w for block p x block q of the factor (as load_a, modes 0 and 1), from clamped addresses.
+__device__ __forceinline__ void load_a_clamped(float (&a)[KS], const DevMat& M, const SweepTab& tab, int p, int q,
+                                               int mode) {
+    const int tp = uni(tab.blk_tile[p]), tq = uni(tab.blk_tile[q]);
+    const int op = uni(tab.blk_off[p]), oq = uni(tab.blk_off[q]);
+    const int wp = min(SB, uni(M.bsz[tp]) - op), wq = min(SB, uni(M.bsz[tq]) - oq);
     const int lane = threadIdx.x & 63;
     const int m = (threadIdx.x >> 6) * 16 + (lane & 15), kg = lane >> 4;
-    const int r = SB * s + m;
-    const TTAD_GLOBAL TL* Tg = gmem(T);
+    const int mc = min(m, wp - 1);
+    if (wp == SB && wq == SB) {  // whole 64 x 64 block (uniform): no condition at all
+        if (mode == 0) {
+            const long ld = M.bsz[tp];
+            const TTAD_GLOBAL float* T = gmem((const float*)M.tiles[tp + (long)tq * M.nt]) + (op + m) + (long)oq * ld;
 #pragma unroll
-    for (int ks = 0; ks < KS; ks++) {
-        const int col = SB * c + 4 * ks + kg;
-        const bool ok = r < bt && col < bt;
-        a[ks] = ok ? (trans ? Tg[col + (long)r * bt] : Tg[r + (long)col * bt]) : TL(0);
+            for (int ks = 0; ks < KS; ks++) a[ks] = T[(long)(4 * ks + kg) * ld];
+        } else {
+            const long ld = M.bsz[tq];
+            const TTAD_GLOBAL float* T = gmem((const float*)M.tiles[tq + (long)tp * M.nt]) + oq + (long)(op + m) * ld;
+#pragma unroll
+            for (int ks = 0; ks < KS; ks++) a[ks] = T[4 * ks + kg];
+        }
+        return;
+    }
+    if (mode == 0) {
+        const long ld = M.bsz[tp];
+        const TTAD_GLOBAL float* T = gmem((const float*)M.tiles[tp + (long)tq * M.nt]) + (op + mc) + (long)oq * ld;
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const int k = 4 * ks + kg;
+            const float v = T[(long)min(k, wq - 1) * ld];
+            a[ks] = (m < wp && k < wq) ? v : 0.0f;
+        }
+    } else {
+        const long ld = M.bsz[tq];
+        const TTAD_GLOBAL float* T = gmem((const float*)M.tiles[tq + (long)tp * M.nt]) + oq + (long)(op + mc) * ld;
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const int k = 4 * ks + kg;
+            const float v = T[min(k, wq - 1)];
+            a[ks] = (m < wp && k < wq) ? v : 0.0f;
+        }
     }
 }
 
-// One 64-row block of a tile-blocked sweep (see the top of this file). forward: Y[p] = rows of L^{-1} R;
-// backward: D[p] = rows of L^{-T} Y, X[p] += D[p]. Rp holds r' (ld ldp). flags[0] counts finished
-// blocks (whole tiles finish together), flags[1 + t] the blocks of tile t that published r'.
-template <typename TL, typename TC, int NR>
-__device__ __forceinline__ void item_tsweep(const DevMat& M, const TileSweepTab& tt_, TL* const* tinv, int forward,
-                                            int p, const double* R, long ldr, TC* Y, long ldy, TC* D, long ldd,
-                                            double* X, long ldx, TC* Rp, long ldp, int nrhs, int32_t* flags,
-                                            int32_t* err, int64_t spin_limit) {
-    constexpr int NB = NR / 16;
-    const SweepTab tab{tt_.P, tt_.blk_tile, tt_.blk_off};
-    const int P = tt_.P;
-    const int t = tt_.blk_tile[p], s = tt_.blk_off[p] / SB;
-    const int bt = M.bsz[t], t0 = tt_.tfirst[t], nbt = tt_.tfirst[t + 1] - t0;
-    const int g0 = M.boff[t] + tt_.blk_off[p], w = min(SB, bt - tt_.blk_off[p]);
+// Waits (thread 0 polls, the barrier passes it on) until *cnt >= want.
+__device__ __forceinline__ void ls_wait(const int32_t* cnt, int want, int32_t* err, int64_t spin_limit, bool slow) {
+    if (threadIdx.x == 0) (void)wait_count(cnt, want, err, spin_limit, slow);
+    __syncthreads();
+}
+
+// One 64-row block p of a lookahead sweep (see the top of this file). flags[t]: blocks of tile t whose r'
+// is published; flags[nt + t]: blocks of tile t whose output (y or x) is final.
+__device__ __forceinline__ void item_lsweep(const DevMat& M, const LSweepTab& lt, int forward, int p, const double* R,
+                                            long ldr, float* Y, float* D, double* X, long ldx, float* Rp, int nrhs,
+                                            int32_t* flags, int32_t* err, int64_t spin_limit, float* As) {
+    const SweepTab tab{lt.P, lt.blk_tile, lt.blk_off};
+    const int nt = lt.nt;
+    const int t = uni(lt.blk_tile[p]), boff_p = uni(lt.blk_off[p]);
+    const int bt = uni(M.bsz[t]), nbt = uni(lt.tfirst[t + 1]) - uni(lt.tfirst[t]);
+    const int tg0 = uni(M.boff[t]), g0 = tg0 + boff_p, w = min(SB, bt - boff_p);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const TC* Src = forward ? Y : D;
-    const long lds_ = forward ? ldy : ldd;
+    const float* Src = forward ? Y : D;
+    int32_t* rcnt = flags;
+    int32_t* ocnt = flags + nt;
+    const bool has_near = forward ? t >= 1 : t + 1 < nt;
+    const int tn = forward ? t - 1 : t + 1;
+
+    // this block's right-hand side (MFMA result layout), off the chain
+    float rhs[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int m = wv * 16 + Mfma<float>::row(lane, r), n = lane & 15;
+        rhs[r] = 0.0f;
+        if (m < w && n < nrhs) rhs[r] = forward ? (float)gmem(R)[(g0 + m) + (long)n * ldr] : gmem(Y)[swz(tg0, boff_p + m, n)];
+    }
+    // the lookahead operand, staged while the far tiles are processed
+    const int btn = has_near ? uni(M.bsz[tn]) : 0, nbn = has_near ? uni(lt.tfirst[tn + 1]) - uni(lt.tfirst[tn]) : 0;
+    if (has_near) stage_rows(As, (forward ? lt.mfac[t] : lt.kfac[t]) + boff_p, bt, w, btn);
+    f4_t acc = {0, 0, 0, 0};
+    float b[LS_KS];
+
+    // A. far tiles (t - 2 and before in sweep order), each once all its outputs are final
+    const int nfar = forward ? max(0, t - 1) : max(0, nt - t - 2);
     const int mode = forward ? 0 : 1;
-    typedef typename Mfma<TC>::acc_t acc_t;
-    acc_t acc[NB];
+    // The matrix operand of a far tile (its 8 blocks, 16 values per lane each) is loaded before waiting for the
+    // tile, the next tile's right after the current products: only the right-hand sides load after a wait.
+    constexpr int NBT = LS_BMAX / SB;
+    float a[NBT][KS];
+    auto load_tile_a = [&](int u) {
+        const int q0 = uni(lt.tfirst[u]), nbu = uni(lt.tfirst[u + 1]) - q0;
 #pragma unroll
-    for (int nb = 0; nb < NB; nb++) acc[nb] = acc_t{0, 0, 0, 0};
-
-    // this block's right-hand side, off the critical path
-    TC rhs[NB][4];
-#pragma unroll
-    for (int nb = 0; nb < NB; nb++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int m = wv * 16 + Mfma<TC>::row(lane, r), n = nb * 16 + (lane & 15);
-            rhs[nb][r] = TC(0);
-            if (m < w && n < nrhs)
-                rhs[nb][r] = forward ? (TC)gmem(R)[(g0 + m) + (long)n * ldr] : gmem(Y)[(g0 + m) + (long)n * ldy];
-        }
-
-    // ---- phase 1: contributions of the earlier tiles (in sweep order), each once it is complete ----
-    const int need = forward ? t0 : P - (t0 + nbt);
-    auto qof = [&](int i) { return forward ? i : P - 1 - i; };
-    // finished-block count at which predecessor i's tile is complete
-    auto want_of = [&](int i) {
-        const int u = tt_.blk_tile[qof(i)];
-        return forward ? tt_.tfirst[u + 1] : P - tt_.tfirst[u];
+        for (int c = 0; c < NBT; c++)
+            if (c < nbu) load_a_clamped(a[c], M, tab, p, q0 + c, mode);
     };
-    __shared__ int s_seen[2];
-    int nwait = 0, seen = 0;
-    // (blocks of one tile finish in any order: only whole-tile counts tell that a block is final)
-    auto wait_for = [&](int i) {
-        const int want = want_of(i);
-        if (want <= seen) return;  // uniform
-        // blocks of tiles before the previous one wait with slow polls
-        const bool far = want < need;
-        if (threadIdx.x == 0) s_seen[nwait & 1] = wait_count(flags, want, err, spin_limit, far);
-        __syncthreads();
-        seen = s_seen[nwait & 1];
-        nwait++;
-    };
+    if (nfar > 0) load_tile_a(forward ? 0 : nt - 1);
+    for (int f = 0; f < nfar; f++) {
+        const int u = forward ? f : nt - 1 - f;
+        const int nbu = uni(lt.tfirst[u + 1]) - uni(lt.tfirst[u]);
+        ls_wait(ocnt + u, nbu, err, spin_limit, f + 1 < nfar);
+        load_rows16(b, Src, uni(M.boff[u]), uni(M.bsz[u]));
+#pragma unroll
+        for (int c = 0; c < NBT; c++)
+            if (c < nbu) {
+#pragma unroll
+                for (int k = 0; k < KS; k++) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[c][k], b[c * KS + k], acc, 0, 0, 0);
+            }
+        if (f + 1 < nfar) load_tile_a(forward ? f + 1 : nt - 2 - f);
+    }
+    // B. the neighbour tile through its published r' (the chain step)
+    if (has_near) {
+        ls_wait(rcnt + tn, nbn, err, spin_limit, false);  // also publishes As
+        load_rows16(b, Rp, uni(M.boff[tn]), btn);
+        lds_tile_mma(acc, As, b);
+    }
+    // 2. publish r'[p]
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int m = wv * 16 + Mfma<float>::row(lane, r), n = lane & 15;
+        if (m < w) gmem(Rp)[swz(tg0, boff_p + m, n)] = rhs[r] - acc[r];
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the stores are in the L2 before the barrier
+    __syncthreads();                     // (also: every wave is done reading As)
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(rcnt + t, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    // C. output rows: Tinv_t[s, :] r'_t (forward) or U_t[s, :] r''_t (backward). The operand is stored in fragment
+    // order (k_tile_inv): 32 loads of 16 bytes per lane, issued before the wait for the rest of r'_t.
     {
-        TL a0[KS], a1[KS];
-        TC b0[NB][KS], b1[NB][KS];
-        bool h0 = false, h1 = false;
-        if (need > 0) load_a<TL>(a0, M, tab, p, qof(0), mode);
-        if (need > 1) load_a<TL>(a1, M, tab, p, qof(1), mode);
-        auto step = [&](int i, TL (&a)[KS], TC (&b)[NB][KS], bool& h) {
-            const int q = qof(i);
-            wait_for(i);
-            TC bc[NB][KS];
-            if (h) {
+        typedef float f4a_t __attribute__((ext_vector_type(4)));
+        const TTAD_GLOBAL f4a_t* T = reinterpret_cast<const TTAD_GLOBAL f4a_t*>(gmem(forward ? lt.tfrag[t] : lt.ufrag[t])) +
+                                     (long)(boff_p / SB * 4 + wv) * (LS_KS / 4) * 64 + lane;
 #pragma unroll
-                for (int nb = 0; nb < NB; nb++)
+        for (int j = 0; j < LS_KS / 4; j++) {
+            const f4a_t v = T[j * 64];
 #pragma unroll
-                    for (int ks = 0; ks < KS; ks++) bc[nb][ks] = b[nb][ks];
+            for (int q = 0; q < 4; q++) a[j / 4][(j % 4) * 4 + q] = v[q];
+        }
+    }
+    ls_wait(rcnt + t, nbt, err, spin_limit, false);
+    load_rows16(b, Rp, tg0, bt);
+    acc = f4_t{0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < NBT; c++)
+#pragma unroll
+        for (int k = 0; k < KS; k++) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[c][k], b[c * KS + k], acc, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int m = wv * 16 + Mfma<float>::row(lane, r), n = lane & 15;
+        if (m < w) {
+            const long gi = g0 + m;
+            if (forward) {
+                gmem(Y)[swz(tg0, boff_p + m, n)] = acc[r];
             } else {
-                load_b<TC, TC, NR>(bc, Src, lds_, blk_g0(M, tab, q), blk_rows(M, tab, q), nrhs);
+                gmem(D)[swz(tg0, boff_p + m, n)] = acc[r];
+                if (n < nrhs) gmem(X)[gi + (long)n * ldx] += (double)acc[r];
             }
-            TL ac[KS];
-#pragma unroll
-            for (int ks = 0; ks < KS; ks++) ac[ks] = a[ks];
-            h = false;
-            if (i + 2 < need) {
-                const int qn = qof(i + 2);
-                load_a<TL>(a, M, tab, p, qn, mode);
-                if (want_of(i + 2) <= seen) {
-                    load_b<TC, TC, NR>(b, Src, lds_, blk_g0(M, tab, qn), blk_rows(M, tab, qn), nrhs);
-                    h = true;
-                }
-            }
-            reg_mma<TC, TL, NB>(acc, ac, bc);
-        };
-        for (int i = 0; i < need; i += 2) {
-            step(i, a0, b0, h0);
-            if (i + 1 < need) step(i + 1, a1, b1, h1);
         }
     }
-
-    // ---- phase 2: publish r'[p] ----
-#pragma unroll
-    for (int nb = 0; nb < NB; nb++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int m = wv * 16 + Mfma<TC>::row(lane, r), n = nb * 16 + (lane & 15);
-            if (m < w && n < nrhs) gmem(Rp)[(g0 + m) + (long)n * ldp] = rhs[nb][r] - acc[nb][r];
-        }
     __builtin_amdgcn_s_waitcnt(0x0F70);
     __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_fetch_add(flags + 1 + t, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-
-    // ---- phase 3: y[p] = Tinv_t[s, :] r'_t (forward, blocks c <= s) or (Tinv_t^T)[s, :] r'_t (c >= s) ----
-    const TL* T = tinv[t];
-    const int c_lo = forward ? 0 : s, c_hi = forward ? s : nbt - 1;
-    const int nc = c_hi - c_lo + 1;
-    TL ta0[KS], ta1[KS];
-    load_tinv<TL>(ta0, T, bt, s, c_lo, !forward);  // constant operands: loaded before the wait
-    if (nc > 1) load_tinv<TL>(ta1, T, bt, s, c_lo + 1, !forward);
-    if (threadIdx.x == 0) (void)wait_count(flags + 1 + t, nbt, err, spin_limit, false);
-    __syncthreads();
-#pragma unroll
-    for (int nb = 0; nb < NB; nb++) acc[nb] = acc_t{0, 0, 0, 0};
-    const int tg0 = M.boff[t];
-    auto step3 = [&](int c, TL (&a)[KS]) {
-        TC bc[NB][KS];
-        load_b<TC, TC, NR>(bc, Rp, ldp, tg0 + SB * c, min(SB, bt - SB * c), nrhs);
-        TL ac[KS];
-#pragma unroll
-        for (int ks = 0; ks < KS; ks++) ac[ks] = a[ks];
-        if (c + 2 <= c_hi) load_tinv<TL>(a, T, bt, s, c + 2, !forward);
-        reg_mma<TC, TL, NB>(acc, ac, bc);
-    };
-    for (int c = c_lo; c <= c_hi; c += 2) {
-        step3(c, ta0);
-        if (c + 1 <= c_hi) step3(c + 1, ta1);
-    }
-#pragma unroll
-    for (int nb = 0; nb < NB; nb++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int m = wv * 16 + Mfma<TC>::row(lane, r), n = nb * 16 + (lane & 15);
-            if (m < w && n < nrhs) {
-                const long gi = g0 + m;
-                if (forward) {
-                    gmem(Y)[gi + (long)n * ldy] = acc[nb][r];
-                } else {
-                    gmem(D)[gi + (long)n * ldd] = acc[nb][r];
-                    gmem(X)[gi + (long)n * ldx] += (double)acc[nb][r];
-                }
-            }
-        }
-    __builtin_amdgcn_s_waitcnt(0x0F70);
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_fetch_add(flags, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(ocnt + t, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// One workgroup per 64-row block, claimed through a ticket counter in sweep order: every block a
-// workgroup waits on (earlier tiles, and the other blocks of its own tile in phase 3) holds a smaller
-// ticket or belongs to its own tile, whose blocks hold consecutive tickets, so it is running or done.
-template <typename TL, typename TC, int NR>
-__global__ __launch_bounds__(NTHREADS) void k_tsweep(const DevMat* Mp, TileSweepTab tab, TL* const* tinv, int forward,
-                                                     const double* R, long ldr, TC* Y, long ldy, TC* D, long ldd,
-                                                     double* X, long ldx, TC* Rp, long ldp, int nrhs, int32_t* flags,
-                                                     int32_t* ticket, int32_t* err, int64_t spin_limit) {
+// One workgroup per 64-row block, claimed through a ticket counter in sweep order: every block a workgroup
+// waits on (earlier tiles, and the blocks of its own tile) holds a smaller ticket or belongs to its own tile,
+// whose blocks hold consecutive tickets, so it is running or done.
+__global__ __launch_bounds__(NTHREADS) void k_lsweep(const DevMat* Mp, LSweepTab tab, int forward, const double* R,
+                                                     long ldr, float* Y, float* D, double* X, long ldx, float* Rp,
+                                                     int nrhs, int32_t* flags, int32_t* ticket, int32_t* err,
+                                                     int64_t spin_limit) {
+    extern __shared__ __align__(16) char lds[];
     __shared__ int s_t;
     if (threadIdx.x == 0) s_t = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     const int q = s_t;
     if (q >= tab.P) return;
     const int p = forward ? q : tab.P - 1 - q;
-    item_tsweep<TL, TC, NR>(*Mp, tab, tinv, forward, p, R, ldr, Y, ldy, D, ldd, X, ldx, Rp, ldp, nrhs, flags, err,
-                            spin_limit);
+    item_lsweep(*Mp, tab, forward, p, R, ldr, Y, D, X, ldx, Rp, nrhs, flags, err, spin_limit,
+                reinterpret_cast<float*>(lds));
 }
 
 }  // namespace ttad
