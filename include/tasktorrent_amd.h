@@ -92,6 +92,13 @@ int ttad_dpotrf(ttad_context* ctx, int n, double* A, int lda, int* info, void* s
 /* cblas_dtrsm(ColMajor, Right, Lower, Trans, NonUnit, m, n, 1.0, L, ldl, B, ldb)
  * (2d_cholesky.cpp:404-405): B := B L^{-T}. */
 int ttad_dtrsm(ttad_context* ctx, int m, int n, const double* L, int ldl, double* B, int ldb, void* stream);
+/* ttad_dtrsm without any allocation: no counterpart in the reference. For callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops). The inverses of L's 64 x 64 diagonal blocks go to a
+ * static per-GPU scratch of 2 MiB, so n <= 4096 (TTAD_ERR_INVALID above that); m is unbounded. Asynchronous on
+ * `stream`; calls that overlap on one GPU share that scratch and race. */
+#define TTAD_DTRSM_NOALLOC_MAX_N 4096
+int ttad_dtrsm_noalloc(ttad_context* ctx, int m, int n, const double* L, int ldl, double* B, int ldb,
+                       void* stream);
 /* cblas_dsyrk(ColMajor, Lower, NoTrans, n, k, alpha, A, lda, beta, C, ldc)
  * (2d_cholesky.cpp:490-491): lower triangle of C := beta C + alpha A A^T. */
 int ttad_dsyrk(ttad_context* ctx, int n, int k, double alpha, const double* A, int lda, double beta, double* C,

@@ -95,6 +95,10 @@ class Context:
         fn = lib().ttad_dtrsm if L.dtype.itemsize == 8 else lib().ttad_strsm
         _capi.check(fn(self._h, m, n, _ptr(L), ldl, _ptr(B), ldb, stream or _stream_of()))
 
+    def trsm_noalloc(self, m, n, L, ldl, B, ldb, stream=None):
+        """FP64 trsm whose diagonal-block inverses live in a static per-GPU scratch (n <= 4096)."""
+        _capi.check(lib().ttad_dtrsm_noalloc(self._h, m, n, _ptr(L), ldl, _ptr(B), ldb, stream or _stream_of()))
+
     def syrk(self, n, k, alpha, A, lda, beta, C, ldc, stream=None):
         fn = lib().ttad_dsyrk if A.dtype.itemsize == 8 else lib().ttad_ssyrk
         _capi.check(fn(self._h, n, k, alpha, _ptr(A), lda, beta, _ptr(C), ldc, stream or _stream_of()))

@@ -93,6 +93,7 @@ _SIGS = {
     "ttad_dpotrf": (I, [V, I, V, I, PI, V]),
     "ttad_spotrf": (I, [V, I, V, I, PI, V]),
     "ttad_dtrsm": (I, [V, I, I, V, I, V, I, V]),
+    "ttad_dtrsm_noalloc": (I, [V, I, I, V, I, V, I, V]),
     "ttad_strsm": (I, [V, I, I, V, I, V, I, V]),
     "ttad_dsyrk": (I, [V, I, I, D, V, I, D, V, I, V]),
     "ttad_ssyrk": (I, [V, I, I, F, V, I, F, V, I, V]),

@@ -26,6 +26,12 @@ int set_error(int code, const char* fmt, ...) {
 
 static hipStream_t pick(ttad_context* c, void* s) { return s ? (hipStream_t)s : c->stream; }
 
+// Scratch of ttad_dtrsm_noalloc: the inverses of the 64 x 64 diagonal blocks of L (FP64, 64 x 64 per 64 columns),
+// for n up to NOALLOC_TRSM_MAX_N: 2 MiB, one copy per GPU, held until the process ends. Host code uses the
+// address that hipGetSymbolAddress returns for the current device (ttad_context::noalloc_dinv), never the name.
+constexpr int NOALLOC_TRSM_MAX_N = 4096;
+static __device__ __align__(256) double scratch[(NOALLOC_TRSM_MAX_N / DB) * DB * DB];
+
 // ---------------------------------------------------------------------------
 // Host-sequenced schedule: per panel step k one launch each for POTRF(k), all
 // TRSM(i,k) and all SYRK/GEMM(k,i,j); stream order carries the dependencies.
@@ -135,6 +141,8 @@ int ttad_init(int device, ttad_context** out) {
     if (e == hipSuccess) e = factor_check_init();
     if (e == hipSuccess) e = solve_init();
     if (e == hipSuccess) e = c->scratch_info.alloc(sizeof(int));
+    // this device's copy of the no-allocation scratch (the device is current here)
+    if (e == hipSuccess) e = hipGetSymbolAddress((void**)&c->noalloc_dinv, HIP_SYMBOL(scratch));
     if (e != hipSuccess) {
         delete c;
         return set_error(TTAD_ERR_DEVICE, "init: %s", hipGetErrorString(e));
@@ -254,6 +262,22 @@ int ttad_spotrf(ttad_context* ctx, int n, float* A, int lda, int* info, void* st
 }
 int ttad_dtrsm(ttad_context* ctx, int m, int n, const double* L, int ldl, double* B, int ldb, void* stream) {
     return trsm_common(ctx, P_F64, m, n, L, ldl, B, ldb, stream);
+}
+// The kernels of ttad_dtrsm with the file-scope scratch: no allocation, no synchronization. Every call on one
+// GPU shares that GPU's scratch, so calls that overlap on one GPU (other streams, other contexts) race.
+int ttad_dtrsm_noalloc(ttad_context* ctx, int m, int n, const double* L, int ldl, double* B, int ldb, void* stream) {
+    if (!ctx || m < 0 || n < 0 || ldl < std::max(1, n) || ldb < std::max(1, m))
+        return set_error(TTAD_ERR_INVALID, "dtrsm_noalloc: bad args");
+    if (n > NOALLOC_TRSM_MAX_N)
+        return set_error(TTAD_ERR_INVALID, "dtrsm_noalloc: n = %d exceeds its scratch limit of %d", n,
+                         NOALLOC_TRSM_MAX_N);
+    if (!ctx->noalloc_dinv) return set_error(TTAD_ERR_DEVICE, "dtrsm_noalloc: no scratch address for this device");
+    if (m == 0 || n == 0) return TTAD_OK;
+    hipStream_t s = pick(ctx, stream);
+    launch_trtri_diag(P_F64, L, ldl, n, ctx->noalloc_dinv, s);  // writes every element it later reads
+    launch_raw_trsm(P_F64, L, ldl, ctx->noalloc_dinv, B, ldb, m, n, s);
+    TTAD_HIP(hipGetLastError());
+    return TTAD_OK;
 }
 int ttad_strsm(ttad_context* ctx, int m, int n, const float* L, int ldl, float* B, int ldb, void* stream) {
     return trsm_common(ctx, P_F32, m, n, L, ldl, B, ldb, stream);

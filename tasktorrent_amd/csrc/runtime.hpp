@@ -66,6 +66,7 @@ struct ttad_context {
     int vset_workers_per_cu[8] = {1, 1, 1, 1, 1, 1, 1, 1};  // per executor instantiation
     ttad::DevBuf scratch_dinv;  // inverse diagonal blocks for the raw tile entry points
     ttad::DevBuf scratch_info;  // one int
+    double* noalloc_dinv = nullptr;  // this device's address of the static scratch of ttad_dtrsm_noalloc
     // host-sequenced schedules, keyed by (matrix id, prec)
     std::map<std::pair<long long, int>, std::shared_ptr<ttad::StreamPlan>> stream_plans;
     // persistent graphs for ttad_potrf_tiled, keyed by (matrix id, prio kind, prec)
