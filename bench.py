@@ -333,6 +333,11 @@ def stress_leg(ctx, rate: float, n_req: int, seed: int = 11, prec: str = "bf16x3
            "solves_per_s": round(n_req / wall, 2), "wall_s": round(wall, 3),
            "max_backward_error": float(max(r["eta"] for r in recs)),
            "max_backward_error_fp64_check": float(np.max(host)),
+           "ir_steps_mean": round(float(np.mean([r["iters"] for r in recs])), 2),
+           "ir_steps_max": int(max(r["iters"] for r in recs)),
+           "ir_steps_mean_by_kappa_decade": {f"1e{d}": round(float(np.mean([recs[q]["iters"] for q in range(n_req)
+                                                                            if int(logk[q]) == d])), 2)
+                                             for d in range(2, 10) if any(int(L) == d for L in logk)},
            "sizes": "n = 256*round(min(16384, Pareto(alpha=1.5, x_m=256))/256), kappa log-uniform [1e2, 1e10]"}
     for A, M in mats:
         M.close()

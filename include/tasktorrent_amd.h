@@ -81,8 +81,13 @@ int ttad_synchronize(ttad_context* ctx, void* stream);
  * TTAD_OPT_FACTOR_RESIDUAL: value != 0 makes ttad_solve_ir and ttad_posv_mixed also compute
  *   ||L L^T - A||_F / ||A||_F of the factor they used (ttad_factor_residual, O(n^3) work done after
  *   the solution is complete and not counted in its times) into ttad_solve_stats.factor_residual;
- *   0 (default) leaves that field at -1. */
+ *   0 (default) leaves that field at -1.
+ * TTAD_OPT_RESIDUAL_ROWS: value != 0 makes every FP64 residual (ttad_residual and the refinement
+ *   steps) use the row-block kernels, which read each stored block of A twice; 0 (default) uses the
+ *   tile-pair kernels, which read A once, wherever the tiles have at most 512 rows and their
+ *   partial sums are smaller than half of A. Both give r = b - A x to rounding. */
 #define TTAD_OPT_FACTOR_RESIDUAL 1
+#define TTAD_OPT_RESIDUAL_ROWS 2
 int ttad_set_option(ttad_context* ctx, int option, long long value);
 
 /* ---- tile kernels: BLAS/LAPACK-shaped, device pointers, column-major ----------- */

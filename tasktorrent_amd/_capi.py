@@ -20,6 +20,7 @@ TTAD_ERR_GRAPH = 7
 TTAD_ERR_BUSY = 8
 TTAD_ERR_NOT_FOUND = 9
 TTAD_OPT_FACTOR_RESIDUAL = 1
+TTAD_OPT_RESIDUAL_ROWS = 2
 
 TTAD_F64, TTAD_F32, TTAD_BF16, TTAD_BF16X3 = 0, 1, 2, 3
 TTAD_POTRF, TTAD_TRSM, TTAD_SYRK, TTAD_GEMM = 0, 1, 2, 3

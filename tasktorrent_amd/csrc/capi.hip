@@ -199,6 +199,9 @@ int ttad_set_option(ttad_context* ctx, int option, long long value) {
         case TTAD_OPT_FACTOR_RESIDUAL:
             ctx->report_factor_residual = value != 0;
             return TTAD_OK;
+        case TTAD_OPT_RESIDUAL_ROWS:
+            ctx->residual_rows = value != 0;
+            return TTAD_OK;
         default:
             return set_error(TTAD_ERR_INVALID, "set_option: unknown option %d", option);
     }

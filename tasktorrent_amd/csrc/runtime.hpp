@@ -79,6 +79,7 @@ struct ttad_context {
     ttad::DevBuf sw_Y, sw_D, sw_P, sw_R, sw_flags, sw_ticket, sw_err, sw_norms, sw_part;
     // options (ttad_set_option)
     bool report_factor_residual = false;
+    bool residual_rows = false;  // FP64 residuals on the row-block kernels (A read twice)
     // working copies of FP64 matrices for ttad_posv_mixed, keyed by (matrix id, storage)
     std::map<std::pair<long long, int>, ttad_matrix*> work_mats;
 };

@@ -7,6 +7,7 @@
 #include <cstring>
 
 #include "runtime.hpp"
+#include "resid_sym.hpp"
 #include "solve_ops.hpp"
 #include "tile_sweep.hpp"
 
@@ -231,6 +232,8 @@ hipError_t solve_init() {
     if (e == hipSuccess)
         e = hipFuncSetAttribute((const void*)k_pair_gemm, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (gemm_item_lds<float, P_F32>()));
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_resid_sym, hipFuncAttributeMaxDynamicSharedMemorySize, LDS_RESID_SYM);
     return e;
 }
 
@@ -319,18 +322,35 @@ static int residual(ttad_context* ctx, ttad_matrix* A, int nrhs, const double* B
         A->anorm_valid = true;
     }
     double worst = 0.0;
-    // column segments per row block: about 4096 workgroups in all
-    const int nseg = std::max(1, std::min(16, 4096 / std::max(1, t.P)));
     const long n = A->n;
-    if ((rc = ensure(ctx->sw_part, (size_t)nseg * n * 32 * 8))) return rc;
+    const int nt = A->nt;
+    // tile-pair pass (A read once, 16 right-hand sides per pass) when the tiles fit its accumulators
+    // and its partial sums, written and read once, cost less than the second read of A
+    const bool sym = !ctx->residual_rows &&
+                     *std::max_element(A->bsz.begin(), A->bsz.end()) <= RSY_MAX_TILE &&
+                     (double)(nt + 1) * n * RSY_NR * 8 <= 0.5 * (double)A->bytes;
+    // otherwise column segments per row block: about 4096 workgroups in all
+    const int nseg = std::max(1, std::min(16, 4096 / std::max(1, t.P)));
+    const long n_pad = n + SB;
+    const size_t part_doubles = sym ? (size_t)(nt + 1) * n * RSY_NR + (size_t)n_pad * RSY_NR : (size_t)nseg * n * 32;
+    if ((rc = ensure(ctx->sw_part, part_doubles * 8))) return rc;
     double* part = ctx->sw_part.as<double>();
-    for (int c0 = 0; c0 < nrhs; c0 += 32) {
-        const int nc = std::min(32, nrhs - c0);
+    const int width = sym ? RSY_NR : 32;
+    for (int c0 = 0; c0 < nrhs; c0 += width) {
+        const int nc = std::min(width, nrhs - c0);
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(128), 0, s, norms, 3 * 32 + 8);
         const double* Bc = B + (long)c0 * ldb;
         const double* Xc = X + (long)c0 * ldx;
         double* Rc = R + (long)c0 * ldr;
-        if (nc <= 16) {
+        if (sym) {
+            double* Xr = part + (size_t)(nt + 1) * n * RSY_NR;
+            hipLaunchKernelGGL(k_rhs_rows16, dim3((unsigned)((n_pad * RSY_NR + 255) / 256)), dim3(256), 0, s, Xc, ldx,
+                               n, n_pad, nc, Xr);
+            hipLaunchKernelGGL(k_resid_sym, dim3(nt * (nt + 1) / 2), dim3(RSY_THREADS), LDS_RESID_SYM, s, Md, Xr,
+                               part, n);
+            hipLaunchKernelGGL(k_resid_sym_reduce, dim3((unsigned)((n * RSY_NR + NTHREADS - 1) / NTHREADS)),
+                               dim3(NTHREADS), 0, s, part, n, nt + 1, Bc, ldb, Xc, ldx, Rc, ldr, nc, norms);
+        } else if (nc <= 16) {
             hipLaunchKernelGGL((k_resid_part<16>), dim3(t.P * nseg), dim3(NTHREADS), 0, s, Md, t, Xc, ldx, part, n,
                                nseg, nc);
             hipLaunchKernelGGL((k_resid_reduce<16>), dim3(t.P), dim3(NTHREADS), 0, s, Md, t, part, n, nseg, Bc, ldb,

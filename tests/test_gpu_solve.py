@@ -67,6 +67,42 @@ def test_residual(ctx, oracle, nrhs):
     assert abs(eta - backward_error(A, X, B)) <= 1e-12 * backward_error(A, X, B)
 
 
+@pytest.mark.parametrize("sizes,nrhs", [([512] * 4, 16), ([256] * 6, 1), ([320, 448, 232], 20),
+                                        ([512, 512, 100], 7)])
+def test_residual_tile_pairs(ctx, oracle, sizes, nrhs):
+    """The tile-pair residual (each stored element of A read once) against the host product and
+    the row-block kernels (TTAD_OPT_RESIDUAL_ROWS). The diagonal tiles' upper triangles hold
+    garbage: both paths must use the stored lower triangle only. Ragged tiles exercise partial
+    64-row blocks; nrhs = 20 takes two 16-column passes. Two calls give the same bits (the
+    partials are added in a fixed order)."""
+    n = sum(sizes)
+    A = oracle.spd_scaled(n, 3, 4)
+    poisoned = A.copy(order="F")
+    poisoned[np.triu_indices(n, 1)] = 1e30
+    M = tt.TiledMatrix(ctx, sizes, "f64")
+    M.set(poisoned)
+    rng = np.random.default_rng(nrhs + n)
+    B = np.asfortranarray(rng.standard_normal((n, nrhs)))
+    X = np.asfortranarray(rng.standard_normal((n, nrhs)))
+    dB, dX = _dev_cols(B), _dev_cols(X)
+    dR1, dR1b, dR2 = (_dev_cols(np.zeros((n, nrhs))) for _ in range(3))
+    eta1 = ctx.residual(M, nrhs, dB, dX, dR1)
+    eta1b = ctx.residual(M, nrhs, dB, dX, dR1b)
+    ctx.set_option(tt._capi.TTAD_OPT_RESIDUAL_ROWS, 1)
+    try:
+        eta2 = ctx.residual(M, nrhs, dB, dX, dR2)
+    finally:
+        ctx.set_option(tt._capi.TTAD_OPT_RESIDUAL_ROWS, 0)
+    M.close()
+    R1, R1b, R2 = from_dev(dR1), from_dev(dR1b), from_dev(dR2)
+    Rref = B - A @ X
+    assert np.linalg.norm(R1 - Rref) / np.linalg.norm(Rref) < 1e-13
+    assert np.linalg.norm(R2 - Rref) / np.linalg.norm(Rref) < 1e-13
+    assert np.array_equal(R1, R1b) and eta1 == eta1b
+    ref_eta = backward_error(A, X, B)
+    assert abs(eta1 - ref_eta) <= 1e-12 * ref_eta and abs(eta2 - ref_eta) <= 1e-12 * ref_eta
+
+
 @pytest.mark.parametrize("prec", ["f32", "bf16x3", "bf16"])
 def test_mixed_ir_reaches_1e12(ctx, oracle, prec):
     n, b, nrhs = 2048, 256, 16
