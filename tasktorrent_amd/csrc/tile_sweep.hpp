@@ -46,11 +46,18 @@ struct LSweepTab {
     float* const* ufrag;      // [nt] U_t likewise
 };
 
+// Order of k in the tile-wide products: MFMA k-step ks of lane group kl (lane / 16) takes k = kperm(ks, kl), so
+// that the four k-steps 4 j .. 4 j + 3 of a lane cover four consecutive k (16 j + 4 kl + 0..3). A product sums
+// over all k, so any order that both operands follow gives it; this one lets a lane load four k-steps of a
+// k-contiguous operand (a transposed factor block, LDS rows) with one 16-byte access.
+__host__ __device__ inline int kperm(int ks, int kl) { return ((ks >> 2) << 4) + (kl << 2) + (ks & 3); }
+
 // Position of element (r, c) of a 512 x 512 operand stored in MFMA A-fragment order: for the 64-row block r / 64,
-// wave (r % 64) / 16 and lane 16 (c % 4) + r % 16, the k-steps ks = c / 4 in groups of four (16 bytes).
+// wave (r % 64) / 16 and lane 16 kl + r % 16 with c = 16 j + 4 kl + q, the k-steps 4 j + q (q = 0..3) together
+// (16 bytes).
 __host__ __device__ inline long fragment_index(int r, int c) {
-    const int ks = c >> 2, lane = ((c & 3) << 4) + (r & 15);
-    return ((((long)(r >> 6) * 4 + ((r >> 4) & 3)) * (LS_KS / 4) + (ks >> 2)) * 64 + lane) * 4 + (ks & 3);
+    const int j = c >> 4, kl = (c >> 2) & 3, q = c & 3, lane = (kl << 4) + (r & 15);
+    return ((((long)(r >> 6) * 4 + ((r >> 4) & 3)) * (LS_KS / 4) + j) * 64 + lane) * 4 + q;
 }
 
 // ---------------------------------------------------------------------------
@@ -192,17 +199,13 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_pair_gemm(const PairGemm* prods
 // on it are scalar and loads under them are not exec-masked.
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
-// B fragments of a tile-wide product: rows [g0, g0 + bu) of a row-major n x 16 FP32 array, k = 4 ks + lane/16,
-// n = lane % 16; rows at or beyond bu read as 0. One load per k-step, all in flight together.
-// Loads are unconditional from clamped (in-bounds) addresses, with out-of-range values selected to 0: no
-// branch separates them, so all of them are in flight together.
 // Layout of the sweeps' row vectors (Y, D, r'; n x 16 FP32): the 16-row groups of each tile hold, per MFMA lane,
 // four consecutive k-steps together, so that a lane loads the B fragments of 4 k-steps with one 16-byte load.
-// Element (k, n) of the tile that starts at row g0 (k tile-local, k = 4 ks + kl) lies at
-// 16 g0 + (((ks / 4) * 64 + 16 kl + n) * 4 + ks % 4); tiles start at multiples of 64 rows.
+// Element (k, n) of the tile that starts at row g0 (k tile-local, k = 16 j + 4 kl + q = kperm(4 j + q, kl)) lies
+// at 16 g0 + ((64 j + 16 kl + n) * 4 + q); tiles start at multiples of 64 rows.
 __device__ __forceinline__ long swz(int g0, int k, int n) {
-    const int ks = k >> 2, kl = k & 3;
-    return (long)g0 * LS_NR + ((((ks >> 2) << 6) + (kl << 4) + n) << 2) + (ks & 3);
+    const int j = k >> 4, kl = (k >> 2) & 3, q = k & 3;
+    return (long)g0 * LS_NR + (((j << 6) + (kl << 4) + n) << 2) + q;
 }
 
 // B fragments of a tile-wide product from a swizzled row vector: the tile at row g0 with bu rows; rows at or
@@ -225,7 +228,7 @@ __device__ __forceinline__ void load_rows16(float (&b)[LS_KS], const float* Y, i
     for (int j = 0; j < LS_KS / 4; j++) {  // the buffer is padded past the last tile: these loads stay in it
         const f4a_t v = p[j * 64];
 #pragma unroll
-        for (int q = 0; q < 4; q++) b[4 * j + q] = 4 * (4 * j + q) + kl < bu ? v[q] : 0.0f;
+        for (int q = 0; q < 4; q++) b[4 * j + q] = 16 * j + 4 * kl + q < bu ? v[q] : 0.0f;
     }
 }
 
@@ -262,18 +265,23 @@ __device__ __forceinline__ void stage_rows(float* As, const float* A, long ld, i
 }
 
 // acc += As(64 x LS_BMAX) * B(LS_BMAX x 16), A from LDS (zero beyond the operand), B fragments in registers
-// (zero beyond the tile); wave w owns rows 16w .. 16w + 15. Four accumulators shorten the chain of dependent
-// MFMAs.
+// (zero beyond the tile); wave w owns rows 16w .. 16w + 15. Four k-steps per 16-byte LDS read (kperm); four
+// accumulators shorten the chain of dependent MFMAs.
 __device__ __forceinline__ void lds_tile_mma(f4_t& acc, const float* As, const float (&b)[LS_KS]) {
     const int lane = threadIdx.x & 63;
-    const float* a = As + ((threadIdx.x >> 6) * 16 + (lane & 15)) * LS_LDK + (lane >> 4);
+    const float* a = As + ((threadIdx.x >> 6) * 16 + (lane & 15)) * LS_LDK + 4 * (lane >> 4);
     f4_t c[4] = {f4_t{0, 0, 0, 0}, f4_t{0, 0, 0, 0}, f4_t{0, 0, 0, 0}, f4_t{0, 0, 0, 0}};
 #pragma unroll
-    for (int ks = 0; ks < LS_KS; ks++) c[ks & 3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[4 * ks], b[ks], c[ks & 3], 0, 0, 0);
+    for (int j = 0; j < LS_KS / 4; j++) {
+        const f4_t v = *reinterpret_cast<const f4_t*>(a + 16 * j);
+#pragma unroll
+        for (int q = 0; q < 4; q++) c[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(v[q], b[4 * j + q], c[q], 0, 0, 0);
+    }
     acc += (c[0] + c[1]) + (c[2] + c[3]);
 }
 
-// A operand of wave w for block p x block q of the factor (as load_a, modes 0 and 1), from clamped addresses.
+// A operand of wave w for block p x block q of the factor (as load_a, modes 0 and 1, with k = kperm(ks, lane / 16)),
+// from clamped addresses. Mode 1 (a transposed block, k contiguous in memory) loads four k-steps per 16 bytes.
 __device__ __forceinline__ void load_a_clamped(float (&a)[KS], const DevMat& M, const SweepTab& tab, int p, int q,
                                                int mode) {
     const int tp = uni(tab.blk_tile[p]), tq = uni(tab.blk_tile[q]);
@@ -287,12 +295,17 @@ __device__ __forceinline__ void load_a_clamped(float (&a)[KS], const DevMat& M, 
             const long ld = M.bsz[tp];
             const TTAD_GLOBAL float* T = gmem((const float*)M.tiles[tp + (long)tq * M.nt]) + (op + m) + (long)oq * ld;
 #pragma unroll
-            for (int ks = 0; ks < KS; ks++) a[ks] = T[(long)(4 * ks + kg) * ld];
+            for (int ks = 0; ks < KS; ks++) a[ks] = T[(long)kperm(ks, kg) * ld];
         } else {
+            typedef float f4u_t __attribute__((ext_vector_type(4), aligned(4)));
             const long ld = M.bsz[tq];
             const TTAD_GLOBAL float* T = gmem((const float*)M.tiles[tq + (long)tp * M.nt]) + oq + (long)(op + m) * ld;
 #pragma unroll
-            for (int ks = 0; ks < KS; ks++) a[ks] = T[4 * ks + kg];
+            for (int j = 0; j < KS / 4; j++) {
+                const f4u_t v = *reinterpret_cast<const TTAD_GLOBAL f4u_t*>(T + 16 * j + 4 * kg);
+#pragma unroll
+                for (int q = 0; q < 4; q++) a[4 * j + q] = v[q];
+            }
         }
         return;
     }
@@ -301,7 +314,7 @@ __device__ __forceinline__ void load_a_clamped(float (&a)[KS], const DevMat& M, 
         const TTAD_GLOBAL float* T = gmem((const float*)M.tiles[tp + (long)tq * M.nt]) + (op + mc) + (long)oq * ld;
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
-            const int k = 4 * ks + kg;
+            const int k = kperm(ks, kg);
             const float v = T[(long)min(k, wq - 1) * ld];
             a[ks] = (m < wp && k < wq) ? v : 0.0f;
         }
@@ -310,7 +323,7 @@ __device__ __forceinline__ void load_a_clamped(float (&a)[KS], const DevMat& M, 
         const TTAD_GLOBAL float* T = gmem((const float*)M.tiles[tq + (long)tp * M.nt]) + oq + (long)(op + mc) * ld;
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
-            const int k = 4 * ks + kg;
+            const int k = kperm(ks, kg);
             const float v = T[min(k, wq - 1)];
             a[ks] = (m < wp && k < wq) ? v : 0.0f;
         }
