@@ -174,7 +174,10 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
         const int64_t e = s_item;
         if (e < 0) break;
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        const int task = (int)(e >> 20), item = (int)(e & 0xFFFFF);
+        // the entry is the same for the whole workgroup (read from LDS): scalar registers, so that the task
+        // record, its tiles and every size derived from them are scalar and their branches uniform
+        const int task = __builtin_amdgcn_readfirstlane((int)(e >> 20));
+        const int item = __builtin_amdgcn_readfirstlane((int)(e & 0xFFFFF));
         const DevTask t = g.tasks[task];
         if (tid == 0) {
             if (t.crit) __hip_atomic_fetch_add(g.cu_crit + my_key, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

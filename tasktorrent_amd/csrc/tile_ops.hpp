@@ -276,9 +276,13 @@ struct GemmNTbf16 {
     }
 
     // full: the whole chunk is inside the operand (uniform), so the loads need no guards; the
-    // guarded form, one exec-masked branch per load, is kept for edge chunks.
+    // guarded form, one exec-masked branch per load, is kept for edge chunks. base is the
+    // operand's (uniform) first row and kfirst is uniform over the wave, so each column's address
+    // is a scalar base plus this lane's 32-bit row offset: no per-lane 64-bit address is kept per
+    // column (those spilled, and a reload from scratch waits for every load in flight, which made
+    // the one-chunk prefetch synchronous).
     template <int G, int GS>
-    static __device__ __forceinline__ void load_raw(float (&r)[G][8], const TA* __restrict__ base, long ld,
+    static __device__ __forceinline__ void load_raw(float (&r)[G][8], const TA* __restrict__ base, long ld, int row,
                                                     bool row_ok, int kfirst, int K, bool full) {
         const TTAD_GLOBAL TA* gb = gmem(base);
         if (full) {
@@ -286,7 +290,7 @@ struct GemmNTbf16 {
             for (int s = 0; s < G; s++) {
                 const int k = kfirst + s * GS * 8;
 #pragma unroll
-                for (int q = 0; q < 8; q++) r[s][q] = (float)gb[(long)(k + q) * ld];
+                for (int q = 0; q < 8; q++) r[s][q] = (float)(gb + (long)(k + q) * ld)[row];
             }
             return;
         }
@@ -294,7 +298,8 @@ struct GemmNTbf16 {
         for (int s = 0; s < G; s++) {
             const int k = kfirst + s * GS * 8;
 #pragma unroll
-            for (int q = 0; q < 8; q++) r[s][q] = (row_ok && k + q < K) ? (float)gb[(long)(k + q) * ld] : 0.0f;
+            for (int q = 0; q < 8; q++)
+                r[s][q] = (row_ok && k + q < K) ? (float)(gb + (long)(k + q) * ld)[row] : 0.0f;
         }
     }
 
@@ -385,11 +390,12 @@ struct GemmNTbf16 {
         unsigned short* S0 = reinterpret_cast<unsigned short*>(lds);
         const int tid = threadIdx.x;
         const int lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
-        // thread -> fixed row (tid % BM) and k-groups of 8 (tid / BM + s * NTHREADS / BM)
-        const int a_row = tid % BM, a_g = tid / BM, b_row = tid % BN, b_g = tid / BN;
+        // thread -> fixed row (tid % BM) and k-groups of 8 (tid / BM + s * NTHREADS / BM); BM and BN are
+        // multiples of the wave size, so a_g and b_g are the same over a wave
+        static_assert(BM % 64 == 0 && BN % 64 == 0, "k-group of a thread must be uniform over its wave");
+        const int a_row = tid % BM, b_row = tid % BN;
+        const int a_g = __builtin_amdgcn_readfirstlane(tid / BM), b_g = __builtin_amdgcn_readfirstlane(tid / BN);
         const bool a_ok = a_row < mv, b_ok = b_row < nv;
-        const TA* Ab = A + a_row;
-        const TA* Bb = B + b_row;
         const int nchunks = (K + BK - 1) / BK;
         __syncthreads();
         if (nchunks == 0) return;
@@ -399,14 +405,14 @@ struct GemmNTbf16 {
             store_planes<GB, NTHREADS / BN>(S + BM * PK, S + (BM + BN) * PK + BM * PK, rb, b_row, b_g);
         };
         // a chunk is full when both operands have all their rows and the chunk lies inside K
-        const bool rows_full = mv >= BM && nv >= BN;
+        const bool rows_full = __builtin_amdgcn_readfirstlane((int)(mv >= BM && nv >= BN)) != 0;
         auto full = [&](int k0) { return rows_full && k0 + BK <= K; };
-        load_raw<GA, NTHREADS / BM>(ra, Ab, lda, a_ok, a_g * 8, K, full(0));
-        load_raw<GB, NTHREADS / BN>(rb, Bb, ldb, b_ok, b_g * 8, K, full(0));
+        load_raw<GA, NTHREADS / BM>(ra, A, lda, a_row, a_ok, a_g * 8, K, full(0));
+        load_raw<GB, NTHREADS / BN>(rb, B, ldb, b_row, b_ok, b_g * 8, K, full(0));
         stage(S0);
         if (nchunks > 1) {
-            load_raw<GA, NTHREADS / BM>(ra, Ab, lda, a_ok, BK + a_g * 8, K, full(BK));
-            load_raw<GB, NTHREADS / BN>(rb, Bb, ldb, b_ok, BK + b_g * 8, K, full(BK));
+            load_raw<GA, NTHREADS / BM>(ra, A, lda, a_row, a_ok, BK + a_g * 8, K, full(BK));
+            load_raw<GB, NTHREADS / BN>(rb, B, ldb, b_row, b_ok, BK + b_g * 8, K, full(BK));
         }
         __syncthreads();
         for (int c = 0; c < nchunks; c++) {
@@ -415,8 +421,8 @@ struct GemmNTbf16 {
                 stage(S0 + ((c + 1) & 1) * STAGE);
                 if (c + 2 < nchunks) {
                     const int k0 = (c + 2) * BK;
-                    load_raw<GA, NTHREADS / BM>(ra, Ab, lda, a_ok, k0 + a_g * 8, K, full(k0));
-                    load_raw<GB, NTHREADS / BN>(rb, Bb, ldb, b_ok, k0 + b_g * 8, K, full(k0));
+                    load_raw<GA, NTHREADS / BM>(ra, A, lda, a_row, a_ok, k0 + a_g * 8, K, full(k0));
+                    load_raw<GB, NTHREADS / BN>(rb, B, ldb, b_row, b_ok, k0 + b_g * 8, K, full(k0));
                 }
             }
             __syncthreads();
@@ -463,23 +469,54 @@ __device__ __forceinline__ void item_gemm(const TS* __restrict__ A, long lda, co
             G::zero(acc);
         }
         G::run(acc, A + r0, lda, mv, B + c0, ldb, nv, K, lds);
+        if constexpr (CPRE) {
 #pragma unroll
-        for (int a = 0; a < G::WM; a++)
+            for (int a = 0; a < G::WM; a++)
 #pragma unroll
-            for (int b = 0; b < G::WN; b++)
+                for (int b = 0; b < G::WN; b++)
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int rr = G::row_of(a, r), cc = G::col_of(b, r);
-                    if (owned(rr, cc)) {
-                        TTAD_GLOBAL TS* p = gmem(C) + (r0 + rr) + (long)(c0 + cc) * ldc;
-                        if constexpr (CPRE) {
-                            *p = (TS)(-acc[a][b][r]);
-                        } else {
-                            const double v = alpha * (double)acc[a][b][r];
-                            *p = (TS)(beta == 0.0 ? v : beta * (double)(*p) + v);
-                        }
+                    for (int r = 0; r < 4; r++) {
+                        const int rr = G::row_of(a, r), cc = G::col_of(b, r);
+                        if (owned(rr, cc)) gmem(C)[(r0 + rr) + (long)(c0 + cc) * ldc] = (TS)(-acc[a][b][r]);
                     }
+        } else {
+            // C is read after the last K-chunk. Every load of a group of fragment rows is issued before any
+            // store of the group: as far as the compiler knows a store may alias a later load, so a load and
+            // a store per element in turn would cost one memory round trip per element (64 per item). Groups:
+            // all of the item for FP32 tiles, one fragment row for FP64 (registers). Loads come from clamped addresses
+            // (C[0] for elements not owned), so none waits under a branch; interior items (all of the
+            // 128 x 128 owned) store without a condition.
+            const bool full = __builtin_amdgcn_readfirstlane(
+                                  (int)(mv == GEMM_BM && nv == GEMM_BM && (!syrk || r0 >= c0 + GEMM_BM))) != 0;
+            constexpr int AG = sizeof(TS) == 8 ? 1 : G::WM;
+#pragma unroll
+            for (int a0 = 0; a0 < G::WM; a0 += AG) {
+                TS cv[AG][G::WN][4];
+                if (beta != 0.0) {
+#pragma unroll
+                    for (int a = 0; a < AG; a++)
+#pragma unroll
+                        for (int b = 0; b < G::WN; b++)
+#pragma unroll
+                            for (int r = 0; r < 4; r++) {
+                                const int rr = G::row_of(a0 + a, r), cc = G::col_of(b, r);
+                                const long off = (r0 + rr) + (long)(c0 + cc) * ldc;
+                                cv[a][b][r] = gmem(C)[(full || owned(rr, cc)) ? off : 0];
+                            }
                 }
+#pragma unroll
+                for (int a = 0; a < AG; a++)
+#pragma unroll
+                    for (int b = 0; b < G::WN; b++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const int rr = G::row_of(a0 + a, r), cc = G::col_of(b, r);
+                            const double v = alpha * (double)acc[a0 + a][b][r];
+                            const TS o = (TS)(beta == 0.0 ? v : beta * (double)cv[a][b][r] + v);
+                            if (full || owned(rr, cc)) gmem(C)[(r0 + rr) + (long)(c0 + cc) * ldc] = o;
+                        }
+            }
+        }
     };
     if constexpr (PREC == P_BF16 || PREC == P_BF16X3) {
         body((GemmNTbf16<TS, GEMM_BM, GEMM_BM, PREC == P_BF16X3, true>*)nullptr);
