@@ -286,12 +286,13 @@ def serving_leg(ctx, rates, n_req: int, seed: int = 7, prec: str = "bf16x3", bud
 
 
 def stress_leg(ctx, rate: float, n_req: int, seed: int = 11, prec: str = "bf16x3", budget: int = 200 << 30):
-    """Config 5: heavy-tailed sizes n = 256*round(min(16384, Pareto(1.5, x_m = 256))/256), nominal kappa
-    10^L with L uniform in [2, 10] through A = D (B B^T/n + delta I) D: half of L from delta (cond of the inner
-    matrix exactly 10^(L/2), B n x n/2), half from D_ii = 10^(L/4 U_i). 1-8 RHS, admission by HBM footprint under
-    a 200 GB arena, refinement capped at 10 steps and then the automatic FP64 refactorization. Reports the
-    fallback rate, the backward errors (the server's and an FP64 recomputation of every request's), the
-    measured condition numbers of the small requests and the latency percentiles."""
+    """Config 5: heavy-tailed sizes n = 256*round(min(16384, Pareto(1.5, x_m = 256))/256), kappa = 10^L with L
+    uniform in [2, 10] through A = D (B B^T/n + delta I) D: delta chosen per request so that the inner matrix
+    has condition number exactly kappa (B n x n/2), D_ii = 10^(U_i / 2) (the scaling of the formula, at most a
+    factor 10 more on cond(A); Cholesky and the refinement are nearly invariant to it). 1-8 RHS, admission by
+    HBM footprint under a 200 GB arena, refinement capped at 10 steps and then the automatic FP64
+    refactorization. Reports the fallback rate, the backward errors (the server's and an FP64 recomputation of
+    every request's), the measured condition numbers of the small requests and the latency percentiles."""
     import numpy as np
     import tasktorrent_amd as tt
     rng = np.random.default_rng(seed)
@@ -302,8 +303,8 @@ def stress_leg(ctx, rate: float, n_req: int, seed: int = 11, prec: str = "bf16x3
         sizes.append(max(256, n))
         logk.append(float(rng.uniform(2.0, 10.0)))
         reqs.append((q, int(rng.integers(1, 9)), -1))
-    kin = [10.0 ** (L / 2) for L in logk]
-    dexps = [L / 4.0 for L in logk]
+    kin = [10.0 ** L for L in logk]
+    dexps = [0.5 for _ in logk]
     mats = _request_matrices(ctx, sizes, [30_000 + q for q in range(n_req)], kin, dexps)
     # the condition numbers the construction reaches, measured (2-norm) on the requests with n <= 1024
     import torch
@@ -335,10 +336,14 @@ def stress_leg(ctx, rate: float, n_req: int, seed: int = 11, prec: str = "bf16x3
            "max_backward_error_fp64_check": float(np.max(host)),
            "ir_steps_mean": round(float(np.mean([r["iters"] for r in recs])), 2),
            "ir_steps_max": int(max(r["iters"] for r in recs)),
+           "fallback_rate_by_kappa_decade": {f"1e{d}": round(float(np.mean([fb[q] for q in range(n_req)
+                                                                          if int(logk[q]) == d])), 3)
+                                             for d in range(2, 10) if any(int(L) == d for L in logk)},
            "ir_steps_mean_by_kappa_decade": {f"1e{d}": round(float(np.mean([recs[q]["iters"] for q in range(n_req)
                                                                             if int(logk[q]) == d])), 2)
                                              for d in range(2, 10) if any(int(L) == d for L in logk)},
-           "sizes": "n = 256*round(min(16384, Pareto(alpha=1.5, x_m=256))/256), kappa log-uniform [1e2, 1e10]"}
+           "sizes": "n = 256*round(min(16384, Pareto(alpha=1.5, x_m=256))/256), cond(B B^T/n + delta I) "
+                    "log-uniform [1e2, 1e10] through delta, D_ii = 10^(U/2)"}
     for A, M in mats:
         M.close()
     return out
