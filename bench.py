@@ -456,7 +456,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cpu-c3", action="store_true", help="CPU baseline at configs 1 and 2 only")
     ap.add_argument("--fp64-only", action="store_true")
-    ap.add_argument("--serve-rates", default="200,800,1600,3200", help="Poisson request rates of the serving leg (1/s)")
+    ap.add_argument("--serve-rates", default="200,800,1600,3200,6400", help="Poisson request rates of the serving leg (1/s)")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--stress-rate", type=float, default=50.0, help="request rate of the config-5 stress leg (1/s)")
