@@ -417,7 +417,7 @@ hipError_t kernels_init() {
     TTAD_SETLDS2((k_raw_potrf<double>), LDS_POTRF);
     TTAD_SETLDS2((k_raw_potrf<float>), LDS_POTRF);
     TTAD_SETLDS2((k_raw_trsm<double, double>), LDS_TRSM);
-    TTAD_SETLDS2((k_raw_trsm<float, float>), LDS_TRSM);
+    TTAD_SETLDS2((k_raw_trsm<float, float>), LDS_TRSM_F32);
 #undef TTAD_SETLDS2
     return e;
 }
@@ -534,7 +534,7 @@ void launch_raw_trsm(int prec, const void* L, long ldl, const double* dinv, void
         hipLaunchKernelGGL((k_raw_trsm<double, double>), dim3(items), dim3(NTHREADS), LDS_TRSM, s,
                            (const double*)L, ldl, dinv, (double*)X, ldx, m, n);
     else
-        hipLaunchKernelGGL((k_raw_trsm<float, float>), dim3(items), dim3(NTHREADS), LDS_TRSM, s, (const float*)L,
+        hipLaunchKernelGGL((k_raw_trsm<float, float>), dim3(items), dim3(NTHREADS), LDS_TRSM_F32, s, (const float*)L,
                            ldl, dinv, (float*)X, ldx, m, n);
 }
 

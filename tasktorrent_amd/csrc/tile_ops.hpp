@@ -435,6 +435,10 @@ struct GemmNTbf16 {
 // the other's MFMAs (FP64, m = n = 16384, K = 512: 43.5 TF/s with one 130 KB workgroup per CU,
 // 53 TF/s with two; 66 TF/s with the 16-byte loads, coalesced C and C read up front).
 template <typename TC> constexpr int gemm_bk() { return sizeof(TC) == 8 ? 16 : 32; }
+// K-chunk of a TRSM item's update from the solved columns: each chunk waits for its loads (issued one chunk
+// ahead), so fewer chunks cost fewer round trips. 64 FP32 columns fit the LDS of the FP32 executors; the FP64
+// ones allow 32.
+template <typename TS> constexpr int trsm_bk() { return sizeof(TS) == 4 ? 64 : 32; }
 
 // ---------------------------------------------------------------------------
 // GEMM / SYRK work item: C[r0:r0+128, c0:c0+128] = beta*C + alpha * A[r0:,:] B[c0:,:]^T
@@ -1028,7 +1032,7 @@ template <typename TS, typename TC>
 __device__ __forceinline__ void item_trsm(const TS* __restrict__ L, long ldl, const double* __restrict__ dinv, TS* __restrict__ X,
                           long ldx, int m, int n, int r0, char* lds) {
     constexpr bool VEC = std::is_same<TS, TC>::value;
-    using G = GemmNT<TS, TS, TC, TRSM_BM, DB, 32, VEC, true>;     // update from the solved columns
+    using G = GemmNT<TS, TS, TC, TRSM_BM, DB, trsm_bk<TS>(), VEC, true>;  // update from the solved columns
     using G3 = GemmNT<TC, TC, TC, TRSM_BM, DB, DB, false, true>;  // times Dinv^T, both operands in LDS
     const int mv = min(TRSM_BM, m - r0);
     if (mv <= 0) return;
@@ -1106,7 +1110,7 @@ constexpr int LDS_POTRF = PotrfGemm<double>::G::LDS_BYTES > 2 * DB * DP * 8 ? Po
 constexpr int LDS_POTRF_ROWS = GemmNT<double, double, double, DB, DB, DB, false, true>::STAGE * 8;
 constexpr int LDS_TRSM = cmax(GemmNT<double, double, double, TRSM_BM, DB, 32, true, true>::LDS_BYTES,
                               GemmNT<double, double, double, TRSM_BM, DB, DB, false, true>::STAGE * 8);
-constexpr int LDS_TRSM_F32 = cmax(GemmNT<float, float, float, TRSM_BM, DB, 32, true, true>::LDS_BYTES,
+constexpr int LDS_TRSM_F32 = cmax(GemmNT<float, float, float, TRSM_BM, DB, trsm_bk<float>(), true, true>::LDS_BYTES,
                                   GemmNT<float, float, float, TRSM_BM, DB, DB, false, true>::STAGE * 4);
 constexpr int LDS_ITEM_MAX =
     cmax(cmax(cmax(LDS_GEMM_F64, LDS_GEMM_F32), cmax(LDS_GEMM_BF16, LDS_POTRF)), cmax(LDS_TRSM, LDS_POTRF_ROWS));
