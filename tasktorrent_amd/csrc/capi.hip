@@ -767,7 +767,10 @@ static int graph_prepare(ttad_graph* g) {
         for (int b = 0; b < g->n_buckets; b++) g->q_base[b + 1] = g->q_base[b] + g->bucket_items[b];
         std::vector<DevMat> dm(g->mats.size());
         for (size_t s = 0; s < g->mats.size(); s++)
-            if (g->mats[s]) dm[s] = g->mats[s]->host_desc;
+            if (g->mats[s]) {
+                dm[s] = g->mats[s]->host_desc;
+                if (g->with_planes && g->mats[s]->tab_planes.p) dm[s].planes = g->mats[s]->tab_planes.as<void*>();
+            }
         const int ne = std::max(1, g->out_ptr[n]);
         TTAD_HIP(g->d_tasks.alloc(n * sizeof(DevTask)));
         TTAD_HIP(g->d_out_ptr.alloc((n + 1) * 4));

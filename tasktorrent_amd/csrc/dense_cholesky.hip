@@ -2,6 +2,8 @@
 // (one rank, non-accumulate mode), written against the Taskflow<K>/Threadpool
 // drop-in API and compiled into a persistent device graph.
 #include <algorithm>
+#include <cstdint>
+#include <cstdlib>
 #include <array>
 #include <memory>
 #include <string>
@@ -11,6 +13,15 @@
 #include "runtime.hpp"
 
 namespace ttad {
+
+// TTAD_PLANES=0 turns the pre-split bf16x3 operands off (for comparisons).
+static bool planes_enabled() {
+    static const bool on = [] {
+        const char* v = getenv("TTAD_PLANES");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
 
 using int2 = std::array<int, 2>;
 using int3 = std::array<int, 3>;
@@ -137,6 +148,39 @@ static int build_cholesky_graph(ttad_context* ctx, int nb, int storage, int prec
     return TTAD_OK;
 }
 
+// The bf16 hi/lo planes of M's off-diagonal tiles (FP32 storage): every tile (i, j), i > j, whose
+// column count bsz[j] is a multiple of 32 gets bsz[i] * bsz[j] * 4 bytes in one slab; the pointer table
+// is null elsewhere. Allocated once per matrix. Returns false (and allocates nothing) when the slab
+// does not fit; the factorization then splits operands on the fly.
+static bool ensure_planes(ttad_matrix* M) {
+    if (M->tab_planes.p) return true;
+    const int nt = M->nt;
+    std::vector<size_t> off((size_t)nt * nt, SIZE_MAX);
+    size_t total = 0;
+    for (int j = 0; j < nt; j++)
+        for (int i = j + 1; i < nt; i++)
+            if (M->bsz[j] > 0 && M->bsz[j] % 32 == 0 && M->bsz[i] > 0) {
+                off[i + (size_t)j * nt] = total;
+                total += ((size_t)M->bsz[i] * M->bsz[j] * 4 + 255) / 256 * 256;
+            }
+    if (total == 0) return false;
+    if (M->planes.alloc(total) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    std::vector<void*> tab((size_t)nt * nt, nullptr);
+    for (size_t q = 0; q < tab.size(); q++)
+        if (off[q] != SIZE_MAX) tab[q] = (char*)M->planes.p + off[q];
+    if (M->tab_planes.alloc(tab.size() * sizeof(void*)) != hipSuccess ||
+        hipMemcpy(M->tab_planes.p, tab.data(), tab.size() * sizeof(void*), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        M->tab_planes.release();
+        M->planes.release();
+        return false;
+    }
+    return true;
+}
+
 int cholesky_persistent(ttad_context* ctx, ttad_matrix* M, int prio_kind, int prec, hipStream_t s, int* info) {
     (void)info;
     const auto key = std::make_tuple(M->id, prio_kind, prec);
@@ -146,6 +190,13 @@ int cholesky_persistent(ttad_context* ctx, ttad_matrix* M, int prio_kind, int pr
         const int rc = build_cholesky_graph(ctx, M->nt, M->storage, prec, prio_kind, &g);
         if (rc != TTAD_OK) return rc;
         ctx->chol_graphs[key] = g;
+        // bf16x3 updates read their operands pre-split (written by the TRSM items of the same run,
+        // which precede every update that reads them in the PTG); accumulate mode keeps the FP32 path
+        if (prec == TTAD_BF16X3 && M->storage == TTAD_F32 && !(prio_kind & TTAD_ACCUMULATE) && planes_enabled() &&
+            ensure_planes(M)) {
+            g->with_planes = true;
+            g->mats_dirty = true;
+        }
     } else {
         g = it->second;
     }

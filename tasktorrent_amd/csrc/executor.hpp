@@ -144,9 +144,9 @@ __device__ __attribute__((noinline)) void gemm_acc_call(const TS* A, long lda, c
 }
 template <typename TS, typename TC, int VM>
 __device__ __attribute__((noinline)) void trsm_call(const TS* L, long ldl, const double* dinv, TS* X, long ldx, int m,
-                                                    int n, int r0, char* lds) {
+                                                    int n, int r0, char* lds, void* planes) {
     __builtin_amdgcn_s_setprio(3);
-    item_trsm<TS, TC>(L, ldl, dinv, X, ldx, m, n, r0, lds);
+    item_trsm<TS, TC>(L, ldl, dinv, X, ldx, m, n, r0, lds, planes);
     __builtin_amdgcn_s_setprio(0);
 }
 
@@ -169,6 +169,7 @@ __device__ __forceinline__ bool exec_item(const DevTask& t, int it, const DevMat
     const DevMat& M = mats[t.mat];
     const int nt = M.nt;
     auto tile = [&](int i, int j) { return M.tiles[i + (long)j * nt]; };
+    auto planes = [&](int i, int j) { return M.planes ? M.planes[i + (long)j * nt] : nullptr; };
     const bool f64s = M.storage == P_F64;
     switch (t.kind) {
         case K_POTRF: {
@@ -206,13 +207,13 @@ __device__ __forceinline__ bool exec_item(const DevTask& t, int it, const DevMat
             if constexpr ((VM & V_F64S) != 0)
                 if (f64s) {
                     trsm_call<double, double, VM>((const double*)tile(j, j), n, M.dinv[j], (double*)tile(i, j), m, m, n,
-                                              it * TRSM_BM, lds);
+                                              it * TRSM_BM, lds, nullptr);
                     return true;
                 }
             if constexpr ((VM & V_F32S) != 0)
                 if (!f64s) {
                     trsm_call<float, float, VM>((const float*)tile(j, j), n, M.dinv[j], (float*)tile(i, j), m, m, n,
-                                            it * TRSM_BM, lds);
+                                            it * TRSM_BM, lds, planes(i, j));
                     return true;
                 }
             return false;
@@ -241,7 +242,7 @@ __device__ __forceinline__ bool exec_item(const DevTask& t, int it, const DevMat
         }                                                                                                          \
     }                                                                                                              \
     item_gemm<TS_, PREC_, CPRE_>((const TS_*)tile(i, k), m, (const TS_*)tile(j, k), n, (TS_*)tile(i, j), m, m, n,  \
-                                 K, r0, c0, syrk, -1.0, 1.0, lds);                                                 \
+                                 K, r0, c0, syrk, -1.0, 1.0, lds, planes(i, k), planes(j, k));                     \
     return true;
             // Products read C after the last K-chunk: measured faster than reading it with the first chunk
             // for FP64 (factorization at n = 32768: 246.7 -> 229.4 ms) and for bf16x3 items (11 %).

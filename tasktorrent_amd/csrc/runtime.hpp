@@ -91,6 +91,8 @@ struct ttad_matrix {
     std::vector<int> bsz, boff;
     std::vector<size_t> tile_off;  // element offset of tile (i,j) (i >= j), index i + j*nt
     ttad::DevBuf tiles, dinv, tab_bsz, tab_boff, tab_tiles, tab_dinv, info, desc, pflags, locks;
+    // bf16 hi/lo planes of the off-diagonal tiles (FP32 storage; allocated by the first bf16x3 factorization)
+    ttad::DevBuf planes, tab_planes;
     size_t elem = 8;
     long long bytes = 0;
     ttad::DevMat host_desc{};
@@ -113,6 +115,7 @@ struct ttad_graph {
     std::vector<int32_t> indegree, seeds, out_ptr, out_idx, q_base, bucket_items;
     std::vector<int32_t> q_home;  // per queue: -1 shared, else the XCD whose workers it feeds first
     std::vector<ttad_matrix*> mats;
+    bool with_planes = false;  // hand the matrices' bf16 planes to the items (a full Cholesky PTG only)
     std::vector<int32_t> spin_us;  // per task: spin time of a TTAD_SPIN task (microseconds)
     bool seeds_dirty = true, mats_dirty = true, running = false;
     ttad::DevBuf d_tasks, d_out_ptr, d_out_idx, d_deps0, d_tail0, d_init, d_init_slot, d_deps, d_items_left, d_q_head, d_q_tail, d_q_count, d_xcd_run,

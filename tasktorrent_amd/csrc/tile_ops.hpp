@@ -27,6 +27,16 @@ template <typename T> __device__ __forceinline__ const TTAD_GLOBAL T* gmem(const
     return (const TTAD_GLOBAL T*)p;
 }
 template <typename T> __device__ __forceinline__ TTAD_GLOBAL T* gmem(T* p) { return (TTAD_GLOBAL T*)p; }
+// A value the whole wave holds, moved to scalar registers: values loaded from task records in global
+// memory are vector loads (the executors write global memory), so the compiler cannot tell they are
+// uniform; address arithmetic on them would otherwise be 64-bit vector work per lane.
+__device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+template <typename T> __device__ __forceinline__ T* wave_uniform(T* p) {
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return (T*)(((unsigned long long)hi << 32) | lo);
+}
 
 // ---------------------------------------------------------------------------
 // MFMA 16x16 primitives. Operand layouts (lane l of the wave):
@@ -334,6 +344,17 @@ struct GemmNTbf16 {
         }
     }
 
+    // 8 consecutive bf16 (one lane's k-group, already split) into LDS
+    static __device__ __forceinline__ void put8(unsigned short* p, u4_t v) {
+        if constexpr (NARROW) {
+            typedef unsigned u2_t __attribute__((ext_vector_type(2)));
+            *reinterpret_cast<u2_t*>(p) = u2_t{v[0], v[1]};
+            *reinterpret_cast<u2_t*>(p + 4) = u2_t{v[2], v[3]};
+        } else {
+            *reinterpret_cast<u4_t*>(p) = v;
+        }
+    }
+
     // 8 consecutive bf16 (one lane's k-group) from LDS
     static __device__ __forceinline__ s8_t ld8(const unsigned short* p) {
         if constexpr (NARROW) {
@@ -428,7 +449,87 @@ struct GemmNTbf16 {
             __syncthreads();
         }
     }
+
+    // As run() for SPLIT with BM x BN fully inside both operands and K % BK == 0, from operands
+    // already split into bf16 planes (plane_granule layout; PA and PB point at the granule of the
+    // item's first row, arows / brows are the tiles' row counts). Each thread moves whole 16-byte
+    // granules, so a chunk costs 4 loads per operand and no conversion; the LDS buffers receive
+    // exactly what run() would store from the FP32 values the planes were split from.
+    static __device__ __forceinline__ void run_planes(acc_t (&acc)[WM][WN], const u4_t* __restrict__ PA, int arows,
+                                                      const u4_t* __restrict__ PB, int brows, int K, char* lds) {
+        static_assert(SPLIT && BK == 32, "plane chunks are 32 columns of hi and lo");
+        unsigned short* S0 = reinterpret_cast<unsigned short*>(lds);
+        const int tid = threadIdx.x;
+        const int lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
+        const int a_row = tid % BM, b_row = tid % BN;
+        const int a_g = __builtin_amdgcn_readfirstlane(tid / BM), b_g = __builtin_amdgcn_readfirstlane(tid / BN);
+        const int nchunks = wave_uniform(K / BK);
+        arows = wave_uniform(arows);
+        brows = wave_uniform(brows);
+        __syncthreads();
+        if (nchunks == 0) return;
+        u4_t ra[GA][2], rb[GB][2];
+        // a uniform granule base per (chunk, plane, group) plus this lane's 32-bit row offset
+        const TTAD_GLOBAL u4_t* ga = gmem(wave_uniform(PA));
+        const TTAD_GLOBAL u4_t* gb = gmem(wave_uniform(PB));
+        auto fetch = [&](int c) {
+#pragma unroll
+            for (int s = 0; s < GA; s++)
+#pragma unroll
+                for (int p = 0; p < 2; p++)
+                    ra[s][p] = (ga + plane_granule(c, p, a_g + s * (NTHREADS / BM), 0, arows))[a_row];
+#pragma unroll
+            for (int s = 0; s < GB; s++)
+#pragma unroll
+                for (int p = 0; p < 2; p++)
+                    rb[s][p] = (gb + plane_granule(c, p, b_g + s * (NTHREADS / BN), 0, brows))[b_row];
+        };
+        auto stage = [&](unsigned short* S) {
+            unsigned short* Ah = S;
+            unsigned short* Bh = S + BM * PK;
+            unsigned short* Al = S + (BM + BN) * PK;
+            unsigned short* Bl = Al + BM * PK;
+#pragma unroll
+            for (int s = 0; s < GA; s++) {
+                const int off = a_row * PK + (a_g + s * (NTHREADS / BM)) * 8;
+                put8(Ah + off, ra[s][0]);
+                put8(Al + off, ra[s][1]);
+            }
+#pragma unroll
+            for (int s = 0; s < GB; s++) {
+                const int off = b_row * PK + (b_g + s * (NTHREADS / BN)) * 8;
+                put8(Bh + off, rb[s][0]);
+                put8(Bl + off, rb[s][1]);
+            }
+        };
+        fetch(0);
+        stage(S0);
+        if (nchunks > 1) fetch(1);
+        __syncthreads();
+        for (int c = 0; c < nchunks; c++) {
+            compute(acc, S0 + (c & 1) * STAGE, lane, wr, wc);
+            if (c + 1 < nchunks) {
+                stage(S0 + ((c + 1) & 1) * STAGE);
+                if (c + 2 < nchunks) fetch(c + 2);
+            }
+            __syncthreads();
+        }
+    }
 };
+
+// Splits 4 consecutive FP32 values of a row (columns k .. k+3, k % 4 == 0) into the bf16 hi/lo planes
+// of a tile of m rows (plane_granule layout), with the rounding GemmNTbf16 applies on its way to LDS.
+__device__ __forceinline__ void store_split4(unsigned short* __restrict__ planes, int m, int row, int k, f4_t x) {
+    typedef unsigned u2_t __attribute__((ext_vector_type(2)));
+    const unsigned h01 = pack_bf16x2(x[0], x[1]), h23 = pack_bf16x2(x[2], x[3]);
+    const unsigned l01 = pack_bf16x2(x[0] - __uint_as_float(h01 << 16), x[1] - __uint_as_float(h01 & 0xffff0000u));
+    const unsigned l23 = pack_bf16x2(x[2] - __uint_as_float(h23 << 16), x[3] - __uint_as_float(h23 & 0xffff0000u));
+    TTAD_GLOBAL unsigned short* g = gmem(planes);
+    const long hi = plane_granule(k / 32, 0, (k % 32) / 8, row, m) * 8 + (k % 8);
+    const long lo = plane_granule(k / 32, 1, (k % 32) / 8, row, m) * 8 + (k % 8);
+    *reinterpret_cast<TTAD_GLOBAL u2_t*>(g + hi) = u2_t{h01, h23};
+    *reinterpret_cast<TTAD_GLOBAL u2_t*>(g + lo) = u2_t{l01, l23};
+}
 
 // K-chunk of the 128x128 FP64/FP32 GEMM items: 16 FP64 / 32 FP32 columns, 65 KB of LDS for the
 // two buffers, so that two workgroups share a CU: one's barriers, loads and C traffic overlap
@@ -448,10 +549,12 @@ template <typename TS> constexpr int trsm_bk() { return sizeof(TS) == 4 ? 64 : 3
 // ---------------------------------------------------------------------------
 // CPRE (the Cholesky update, alpha = -1, beta = 1): the accumulators start from -C, read
 // together with the first K-chunk instead of after the last one, and C = -acc is stored.
+// PA / PB (bf16x3 only; null = none): the bf16 split planes of the operand tiles (A and B with lda and ldb
+// rows, plane_granule layout), used by items that lie wholly inside both with K % 32 == 0.
 template <typename TS, int PREC, bool CPRE = false>
 __device__ __forceinline__ void item_gemm(const TS* __restrict__ A, long lda, const TS* __restrict__ B, long ldb, TS* __restrict__ C,
                           long ldc, int m, int n, int K, int r0, int c0, bool syrk, double alpha, double beta,
-                          char* lds) {
+                          char* lds, const void* PA = nullptr, const void* PB = nullptr) {
     const int mv = min(GEMM_BM, m - r0), nv = min(GEMM_BM, n - c0);
     if (mv <= 0 || nv <= 0) return;
     auto body = [&](auto* gsel) {
@@ -472,7 +575,15 @@ __device__ __forceinline__ void item_gemm(const TS* __restrict__ A, long lda, co
         } else {
             G::zero(acc);
         }
-        G::run(acc, A + r0, lda, mv, B + c0, ldb, nv, K, lds);
+        bool planes = false;
+        if constexpr (PREC == P_BF16X3)
+            planes = PA && PB && mv == GEMM_BM && nv == GEMM_BM && K % G::BK == 0;
+        if constexpr (PREC == P_BF16X3) {
+            if (planes)
+                G::run_planes(acc, static_cast<const u4_t*>(PA) + r0, (int)lda, static_cast<const u4_t*>(PB) + c0,
+                              (int)ldb, K, lds);
+        }
+        if (!planes) G::run(acc, A + r0, lda, mv, B + c0, ldb, nv, K, lds);
         if constexpr (CPRE) {
 #pragma unroll
             for (int a = 0; a < G::WM; a++)
@@ -1028,9 +1139,11 @@ __device__ __forceinline__ void item_potrf_rows(TS* __restrict__ A, long lda, in
 // memory between the two products). TC is the MFMA type (FP64, or FP32 for the
 // mixed variants).
 // ---------------------------------------------------------------------------
+// planes (FP32 tiles; null = none): the bf16 split planes of X (ldx rows, plane_granule layout), written
+// beside X for the bf16x3 updates that read it; needs n % 32 == 0.
 template <typename TS, typename TC>
 __device__ __forceinline__ void item_trsm(const TS* __restrict__ L, long ldl, const double* __restrict__ dinv, TS* __restrict__ X,
-                          long ldx, int m, int n, int r0, char* lds) {
+                          long ldx, int m, int n, int r0, char* lds, void* planes = nullptr) {
     constexpr bool VEC = std::is_same<TS, TC>::value;
     using G = GemmNT<TS, TS, TC, TRSM_BM, DB, trsm_bk<TS>(), VEC, true>;  // update from the solved columns
     using G3 = GemmNT<TC, TC, TC, TRSM_BM, DB, DB, false, true>;  // times Dinv^T, both operands in LDS
@@ -1092,6 +1205,19 @@ __device__ __forceinline__ void item_trsm(const TS* __restrict__ L, long ldl, co
                     const int rr = G3::row_of(a, r), cc = G3::col_of(b, r);
                     if (rr < mv && cc < cw) T[rr + (long)cc * ldx] = (TS)acc[a][b][r];
                 }
+        if constexpr (std::is_same<TS, float>::value && std::is_same<TC, float>::value) {
+            // a lane holds 4 consecutive columns of one row per fragment (cw is a multiple of 32 here)
+            if (planes) {
+#pragma unroll
+                for (int a = 0; a < G::WM; a++)
+#pragma unroll
+                    for (int b = 0; b < G::WN; b++) {
+                        const int rr = G3::row_of(a, 0), cc = G3::col_of(b, 0);
+                        if (rr < mv && cc < cw)
+                            store_split4(static_cast<unsigned short*>(planes), (int)ldx, r0 + rr, c0 + cc, acc[a][b]);
+                    }
+            }
+        }
         __syncthreads();
     }
 }

@@ -27,7 +27,16 @@ struct DevMat {
     int32_t* info;         // first non-positive pivot, 1-based global column; 0 = none
     int32_t* pflags;       // [nt * 128]: progress flags of the row-block POTRF items of each diagonal tile (zero between runs)
     int32_t* locks;        // [nt * nt * 16]: accumulate-mode sub-tile locks of tile (i, j) (zero when free)
+    // [nt*nt] or null: bf16 hi/lo planes of the off-diagonal FP32 tiles (i > j, bsz[j] % 32 == 0; other
+    // entries null), written by the TRSM items that produce the tile and read by the bf16x3 GEMM items.
+    // Tile (i, j) with m = bsz[i] rows: 16-byte granule ((c*2 + p)*4 + g)*m + r holds bf16 plane p
+    // (0: hi = bf16(x), 1: lo = bf16(x - hi), both rounded to nearest even) of columns 32c + 8g ... +7 of row r.
+    void* const* planes;
 };
+// bf16 split planes: granule of (chunk of 32 columns, plane, 8-column group, row) in a tile of m rows.
+__host__ __device__ inline long plane_granule(int c, int p, int g, int r, int m) {
+    return ((long)(c * 2 + p) * 4 + g) * m + r;
+}
 
 // Device task record.
 struct DevTask {

@@ -123,6 +123,30 @@ def test_mixed_factor_close(ctx, oracle, prec):
     assert np.array_equal(L, L2)
 
 
+@pytest.mark.parametrize("sizes", [[512] * 4, [512, 320, 448, 96, 384, 100, 256]])
+def test_bf16x3_split_operands_bitwise(ctx, oracle, sizes):
+    """The persistent bf16x3 factorization reads its update operands split once into bf16 hi/lo
+    planes by the TRSM items (tiles whose column count is a multiple of 32, items wholly inside both
+    operands); the host-sequenced schedule splits them on the fly. Same factor bit for bit, also for
+    ragged tiles that keep the on-the-fly path and for a second run on the same matrix (planes reused)."""
+    n = sum(sizes)
+    A = oracle.spd_bbt(n, 41)
+    M = tt.TiledMatrix(ctx, sizes, "f32")
+    try:
+        runs = []
+        for _ in range(2):
+            M.set(A)
+            assert ctx.potrf_tiled(M, tt.TTAD_SCHED_PERSISTENT, 2, "bf16x3") == 0
+            runs.append(M.get())
+    finally:
+        M.close()
+    L2, info = factor(ctx, A, sizes, tt.TTAD_SCHED_STREAM, prec="bf16x3")
+    assert info == 0
+    assert np.array_equal(runs[0], L2)
+    assert np.array_equal(runs[1], L2)
+    assert rel(runs[0], np.linalg.cholesky(A)) < 1e-4
+
+
 def _longest_path_tasks(indeg_edges, n):
     out_ptr, out_idx = indeg_edges
     pending = np.bincount(out_idx, minlength=n)
