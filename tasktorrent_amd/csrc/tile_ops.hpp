@@ -603,6 +603,46 @@ __device__ __forceinline__ void item_gemm(const TS* __restrict__ A, long lda, co
             // 128 x 128 owned) store without a condition.
             const bool full = __builtin_amdgcn_readfirstlane(
                                   (int)(mv == GEMM_BM && nv == GEMM_BM && (!syrk || r0 >= c0 + GEMM_BM))) != 0;
+            if constexpr (sizeof(TS) == 4) {
+                // Interior FP32 items go through LDS (free after the K loop): a lane's results lie along a row,
+                // C is column-major, so the accumulators are transposed in LDS and every thread then reads,
+                // updates and writes 16 contiguous bytes of C per access (16 loads and 16 stores per thread
+                // instead of 64 each). C is read first, so its latency overlaps the transposition.
+                if (full) {
+                    constexpr int P = GEMM_BM + 4;  // LDS column pitch: the 4 column groups of a wave hit distinct banks
+                    static_assert(GEMM_BM * P * 4 <= G::LDS_BYTES, "transposed item fits the item's LDS");
+                    typedef typename V16<TS>::u f4u;
+                    TS* T = reinterpret_cast<TS*>(lds);
+                    const int rg = threadIdx.x % 32, cl = threadIdx.x / 32;  // rows 4rg .. 4rg+3 of columns cl + 8j
+                    const long ldu = wave_uniform((int)ldc);
+                    TTAD_GLOBAL TS* base = gmem(C) + (r0 + 4 * rg) + (long)(c0 + cl) * ldu;
+                    constexpr int NJ = GEMM_BM / 8;
+                    f4_t cv[NJ];
+                    if (beta != 0.0) {
+#pragma unroll
+                        for (int j = 0; j < NJ; j++) cv[j] = *reinterpret_cast<TTAD_GLOBAL f4u*>(base + 8 * j * ldu);
+                    }
+#pragma unroll
+                    for (int a = 0; a < G::WM; a++)
+#pragma unroll
+                        for (int b = 0; b < G::WN; b++)
+#pragma unroll
+                            for (int r = 0; r < 4; r++) T[G::col_of(b, r) * P + G::row_of(a, r)] = acc[a][b][r];
+                    __syncthreads();
+#pragma unroll
+                    for (int j = 0; j < NJ; j++) {
+                        const f4_t v = *reinterpret_cast<const f4_t*>(T + (cl + 8 * j) * P + 4 * rg);
+                        f4_t o;
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const double x = alpha * (double)v[r];
+                            o[r] = (TS)(beta == 0.0 ? x : beta * (double)cv[j][r] + x);
+                        }
+                        *reinterpret_cast<TTAD_GLOBAL f4u*>(base + 8 * j * ldu) = o;
+                    }
+                    return;
+                }
+            }
             constexpr int AG = sizeof(TS) == 8 ? 1 : G::WM;
 #pragma unroll
             for (int a0 = 0; a0 < G::WM; a0 += AG) {
