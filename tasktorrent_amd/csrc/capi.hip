@@ -691,6 +691,7 @@ int ttad_graph_create(ttad_context* ctx, int n_tasks, const ttad_task_desc* task
             g->q_home.push_back(h);
         }
     g->n_buckets = (int)g->q_home.size();
+    if (g->n_buckets > MAX_QUEUES) return set_error(TTAD_ERR_INVALID, "graph_create: %d work queues", g->n_buckets);
     for (int t = 0; t < n_tasks; t++) g->tasks[t].bucket = qid[(size_t)pb[t] * (NXCD + 1) + (home[t] + 1)];
     g->seeds.assign(n_tasks, 0);
     TTAD_HIP(hipEventCreate(&g->ev_start));

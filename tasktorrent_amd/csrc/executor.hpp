@@ -46,6 +46,10 @@ constexpr int TRACE_WORDS = 6;  // u64 per trace record
 
 // XCDs of the device (MI355X: 8 XCDs of 32 CUs, each with its own L2).
 constexpr int NXCD = 8;
+// Work queues of a graph: at most 64 priority buckets, each with a shared queue and one per XCD;
+// the executors scan them in groups of 64 (one queue per lane of a wave).
+constexpr int MAX_QUEUES = 64 * (NXCD + 1);
+constexpr int MAX_QGROUPS = MAX_QUEUES / 64;
 // Hardware id of the XCD (XCC) this wave runs on.
 __device__ __forceinline__ int xcc_id() {
     unsigned v;

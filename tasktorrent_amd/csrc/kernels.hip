@@ -1,4 +1,5 @@
 // HIP kernels of the tiled Cholesky library (gfx950).
+#include <climits>
 #include <algorithm>
 
 #include "kernels.hpp"
@@ -86,6 +87,11 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
     const int my_key = s_key;
     if (primary && tid == 0) __hip_atomic_fetch_add(g.idle_primary, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t rng = blockIdx.x * 2654435761u + 0x9e3779b9u;
+    // the queues' home XCDs, read by every scan, kept in LDS
+    __shared__ signed char s_qhome[MAX_QUEUES];
+    const int ngroups = (g.n_buckets + 63) / 64;
+    for (int b = tid; b < g.n_buckets; b += NTHREADS) s_qhome[b] = (signed char)g.q_home[b];
+    __syncthreads();
     for (;;) {
         if (tid < 64) {
             // wave 0: ballots over groups of 64 buckets (lane = bucket), highest priority first
@@ -97,25 +103,36 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
                 bool any_open = false;
                 // a secondary leaves queues that the idle primaries can drain; nothing is taken while
                 // the other worker of this CU runs an item of the critical path
+                // every load of the scan is issued before any is used: one memory round trip for all groups
                 const int reserve = primary ? 0 : ld_relaxed(g.idle_primary);
                 const bool blocked = ld_relaxed(g.cu_crit + my_key) != 0;
-                for (int base = 0; base < g.n_buckets; base += 64) {  // groups of 64 queues, best first
-                    const int b = base + tid;
-                    bool ne = false, open = false, mine = false;
-                    if (b < g.n_buckets) {
-                        const int h = ld_relaxed(g.q_head + b);
-                        ne = !blocked && ld_relaxed(g.q_count + b) > reserve;
-                        open = h < g.q_base[b + 1] - g.q_base[b];
-                        const int home = g.q_home[b];
-                        mine = ne && (home < 0 || home == my_xcd);
-                    }
+                int cnt[MAX_QGROUPS];
+#pragma unroll
+                for (int q = 0; q < MAX_QGROUPS; q++) {
+                    const int b = q * 64 + tid;
+                    cnt[q] = (q < ngroups && b < g.n_buckets) ? ld_relaxed(g.q_count + b) : INT_MIN;
+                }
+#pragma unroll
+                for (int q = 0; q < MAX_QGROUPS; q++) {  // groups of 64 queues, best first
+                    if (q >= ngroups || fb >= 0) break;
+                    const bool ne = !blocked && cnt[q] > reserve;
+                    const int home = s_qhome[(q * 64 + tid) % MAX_QUEUES];
+                    const bool mine = ne && (home < 0 || home == my_xcd);
                     const unsigned long long mask = __ballot(mine), any = __ballot(ne);
-                    any_open |= __ballot(open) != 0;
-                    if (sb < 0 && any) sb = base + __ffsll((long long)any) - 1;
-                    if (mask) {
-                        fb = base + __ffsll((long long)mask) - 1;
-                        break;
+                    if (sb < 0 && any) sb = q * 64 + __ffsll((long long)any) - 1;
+                    if (mask) fb = q * 64 + __ffsll((long long)mask) - 1;
+                }
+                if (fb < 0 && sb < 0) {  // nothing claimable: is anything left to claim at all?
+                    int h[MAX_QGROUPS], size[MAX_QGROUPS];
+#pragma unroll
+                    for (int q = 0; q < MAX_QGROUPS; q++) {
+                        const int b = q * 64 + tid;
+                        const bool ok = q < ngroups && b < g.n_buckets;
+                        h[q] = ok ? ld_relaxed(g.q_head + b) : 0;
+                        size[q] = ok ? g.q_base[b + 1] - g.q_base[b] : 0;
                     }
+#pragma unroll
+                    for (int q = 0; q < MAX_QGROUPS; q++) any_open |= __ballot(h[q] < size[q]) != 0;
                 }
                 if (fb < 0 && sb >= 0) fb = sb;  // nothing for this XCD: steal
                 const bool all_claimed = fb < 0 && !any_open;
