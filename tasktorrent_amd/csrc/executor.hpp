@@ -50,6 +50,7 @@ constexpr int NXCD = 8;
 // the executors scan them in groups of 64 (one queue per lane of a wave).
 constexpr int MAX_QUEUES = 64 * (NXCD + 1);
 constexpr int MAX_QGROUPS = MAX_QUEUES / 64;
+constexpr int CLAIM_TRIES = 4;  // queues a claimer tries from one scan
 // Hardware id of the XCD (XCC) this wave runs on.
 __device__ __forceinline__ int xcc_id() {
     unsigned v;

@@ -112,6 +112,10 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
                     const int b = q * 64 + tid;
                     cnt[q] = (q < ngroups && b < g.n_buckets) ? ld_relaxed(g.q_count + b) : INT_MIN;
                 }
+                // the candidates: the non-empty queues of the first group that has one for this XCD (else of
+                // the first group with any, to steal from), tried best first
+                unsigned long long cmask = 0, smask = 0;
+                int cq = -1, sq = -1;
 #pragma unroll
                 for (int q = 0; q < MAX_QGROUPS; q++) {  // groups of 64 queues, best first
                     if (q >= ngroups || fb >= 0) break;
@@ -119,8 +123,16 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
                     const int home = s_qhome[(q * 64 + tid) % MAX_QUEUES];
                     const bool mine = ne && (home < 0 || home == my_xcd);
                     const unsigned long long mask = __ballot(mine), any = __ballot(ne);
-                    if (sb < 0 && any) sb = q * 64 + __ffsll((long long)any) - 1;
-                    if (mask) fb = q * 64 + __ffsll((long long)mask) - 1;
+                    if (sb < 0 && any) {
+                        sb = q * 64 + __ffsll((long long)any) - 1;
+                        smask = any;
+                        sq = q;
+                    }
+                    if (mask) {
+                        fb = q * 64 + __ffsll((long long)mask) - 1;
+                        cmask = mask;
+                        cq = q;
+                    }
                 }
                 if (fb < 0 && sb < 0) {  // nothing claimable: is anything left to claim at all?
                     int h[MAX_QGROUPS], size[MAX_QGROUPS];
@@ -134,17 +146,30 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
 #pragma unroll
                     for (int q = 0; q < MAX_QGROUPS; q++) any_open |= __ballot(h[q] < size[q]) != 0;
                 }
-                if (fb < 0 && sb >= 0) fb = sb;  // nothing for this XCD: steal
+                if (fb < 0 && sb >= 0) {  // nothing for this XCD: steal
+                    fb = sb;
+                    cmask = smask;
+                    cq = sq;
+                }
                 const bool all_claimed = fb < 0 && !any_open;
                 if (fb >= 0) {
                     int64_t e = -1;
                     if (tid == 0) {
                         int slot = -1;
-                        if (__hip_atomic_fetch_add(g.q_count + fb, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >
-                            reserve)
-                            slot = __hip_atomic_fetch_add(g.q_head + fb, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        else  // no credit left for this worker: give it back
-                            __hip_atomic_fetch_add(g.q_count + fb, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        // a credit lost to another worker: the next candidate at once (one atomic) rather than
+                        // a new scan
+                        for (int attempt = 0; attempt < CLAIM_TRIES && cmask != 0; attempt++) {
+                            const int b = cq * 64 + __ffsll((long long)cmask) - 1;
+                            cmask &= cmask - 1;
+                            if (__hip_atomic_fetch_add(g.q_count + b, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >
+                                reserve) {
+                                slot = __hip_atomic_fetch_add(g.q_head + b, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                fb = b;
+                                break;
+                            }
+                            // no credit left for this worker: give it back
+                            __hip_atomic_fetch_add(g.q_count + b, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        }
                         if (slot >= 0) {
                             const int64_t* sp = g.q_items + g.q_base[fb] + slot;
                             // claimed before its producer wrote it: poll relaxed (an acquire load would
