@@ -667,7 +667,7 @@ int ttad_graph_create(ttad_context* ctx, int n_tasks, const ttad_task_desc* task
     std::sort(uniq.begin(), uniq.end(), std::greater<double>());
     uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
     const int nu = (int)uniq.size();
-    const int cap = 64;
+    const int cap = QUEUE_BUCKETS;
     const int n_prio = std::min(nu, cap);
     std::vector<int> pb(n_tasks), home(n_tasks, -1);
     for (int t = 0; t < n_tasks; t++) {
@@ -679,20 +679,14 @@ int ttad_graph_create(ttad_context* ctx, int n_tasks, const ttad_task_desc* task
         // spin tasks are homed like GEMM/SYRK (by their row index), so that tiny tasks spread their claims
         if (xcd_homed(x.kind, x.prec)) home[t] = x.i % NXCD;
     }
-    // work queues in priority order; within a bucket the shared queue, then one per home XCD
-    std::vector<int> qid((size_t)n_prio * (NXCD + 1), -1);
-    for (int t = 0; t < n_tasks; t++) qid[(size_t)pb[t] * (NXCD + 1) + (home[t] + 1)] = 0;
-    g->q_home.clear();
-    for (int b = 0; b < n_prio; b++)
-        for (int h = -1; h < NXCD; h++) {
-            int& q = qid[(size_t)b * (NXCD + 1) + (h + 1)];
-            if (q < 0) continue;
-            q = (int)g->q_home.size();
-            g->q_home.push_back(h);
-        }
-    g->n_buckets = (int)g->q_home.size();
-    if (g->n_buckets > MAX_QUEUES) return set_error(TTAD_ERR_INVALID, "graph_create: %d work queues", g->n_buckets);
-    for (int t = 0; t < n_tasks; t++) g->tasks[t].bucket = qid[(size_t)pb[t] * (NXCD + 1) + (home[t] + 1)];
+    // work queues: bucket b's shared queue is shared_queue(b), its queue homed on XCD x homed_queue(x, b)
+    // (the executors scan the shared and own-XCD queues by bucket); the other ids stay empty
+    static_assert(QUEUE_BUCKETS >= 64, "a bucket per priority level");
+    g->q_home.assign(MAX_QUEUES, -1);
+    for (int x = 0; x < NXCD; x++)
+        for (int b = 0; b < QUEUE_BUCKETS; b++) g->q_home[homed_queue(x, b)] = x;
+    g->n_buckets = MAX_QUEUES;
+    for (int t = 0; t < n_tasks; t++) g->tasks[t].bucket = home[t] < 0 ? shared_queue(pb[t]) : homed_queue(home[t], pb[t]);
     g->seeds.assign(n_tasks, 0);
     TTAD_HIP(hipEventCreate(&g->ev_start));
     TTAD_HIP(hipEventCreate(&g->ev_stop));
@@ -790,7 +784,7 @@ static int graph_prepare(ttad_graph* g) {
         TTAD_HIP(g->d_done.alloc(4));
         TTAD_HIP(g->d_abort.alloc(4));
         TTAD_HIP(g->d_trace_cursor.alloc(4));
-        TTAD_HIP(g->d_cu_slots.alloc((2 * CU_KEYS + 1) * 4));  // slots, critical counts, idle primaries
+        TTAD_HIP(g->d_cu_slots.alloc((2 * CU_KEYS + NXCD * 64) * 4));  // slots, critical counts, idle primaries
         TTAD_HIP(g->d_mats.alloc(dm.size() * sizeof(DevMat)));
         TTAD_HIP(hipMemcpy(g->d_tasks.p, g->tasks.data(), n * sizeof(DevTask), hipMemcpyHostToDevice));
         TTAD_HIP(hipMemcpy(g->d_out_ptr.p, g->out_ptr.data(), (n + 1) * 4, hipMemcpyHostToDevice));

@@ -36,7 +36,7 @@ struct DevGraph {
                                 // t_end of the successor release (the task's last item; else 0), 0}
     int32_t* trace_cursor;
     int32_t* cu_slots;          // [CU_KEYS] workers started per CU (slot 0 = primary)
-    int32_t* idle_primary;      // primary workers currently looking for work
+    int32_t* idle_primary;      // [NXCD * 64] primary workers currently looking for work, per XCD (entry 64 * x)
     int32_t* cu_crit;           // [CU_KEYS] critical items running per CU (its other worker waits)
     int32_t* xcd_run;           // [n_tasks * NXCD] items of a task running on each XCD (release elision)
     int64_t spin_limit;         // watchdog in wall_clock64 ticks
@@ -46,10 +46,18 @@ constexpr int TRACE_WORDS = 6;  // u64 per trace record
 
 // XCDs of the device (MI355X: 8 XCDs of 32 CUs, each with its own L2).
 constexpr int NXCD = 8;
-// Work queues of a graph: at most 64 priority buckets, each with a shared queue and one per XCD;
-// the executors scan them in groups of 64 (one queue per lane of a wave).
-constexpr int MAX_QUEUES = 64 * (NXCD + 1);
-constexpr int MAX_QGROUPS = MAX_QUEUES / 64;
+// Work queues of a graph: 64 priority buckets, each with a shared queue and one queue homed on each
+// XCD. Bucket b's shared queue is queue b * QUEUE_SHARED_STRIDE (every worker claims from the shared
+// queues: their counters get a cache line each), its queue homed on XCD x is
+// QUEUE_HOMED + x * QUEUE_BUCKETS + b (only that XCD's workers and stealers claim there: the 64 counters
+// of one XCD share 256 bytes and no line with another XCD's). The ids in between are empty queues.
+// A scan reads one bucket per lane of a wave.
+constexpr int QUEUE_BUCKETS = 64;
+constexpr int QUEUE_SHARED_STRIDE = 32;  // 128-byte lines of 4-byte counters
+constexpr int QUEUE_HOMED = QUEUE_BUCKETS * QUEUE_SHARED_STRIDE;
+constexpr int MAX_QUEUES = QUEUE_HOMED + QUEUE_BUCKETS * NXCD;
+__host__ __device__ constexpr int shared_queue(int b) { return b * QUEUE_SHARED_STRIDE; }
+__host__ __device__ constexpr int homed_queue(int x, int b) { return QUEUE_HOMED + x * QUEUE_BUCKETS + b; }
 constexpr int CLAIM_TRIES = 4;  // queues a claimer tries from one scan
 // Hardware id of the XCD (XCC) this wave runs on.
 __device__ __forceinline__ int xcc_id() {

@@ -85,82 +85,92 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
     __syncthreads();
     const bool primary = s_primary != 0;
     const int my_key = s_key;
-    if (primary && tid == 0) __hip_atomic_fetch_add(g.idle_primary, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // idle primaries are counted per XCD (counters 256 bytes apart): the queues a worker mostly claims
+    // from are those of its XCD
+    int32_t* idle_primary = g.idle_primary + my_xcd * 64;
+    if (primary && tid == 0) __hip_atomic_fetch_add(idle_primary, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t rng = blockIdx.x * 2654435761u + 0x9e3779b9u;
-    // the queues' home XCDs, read by every scan, kept in LDS
-    __shared__ signed char s_qhome[MAX_QUEUES];
-    const int ngroups = (g.n_buckets + 63) / 64;
-    for (int b = tid; b < g.n_buckets; b += NTHREADS) s_qhome[b] = (signed char)g.q_home[b];
-    __syncthreads();
+    const int own_q = homed_queue(my_xcd, 0);  // first queue homed on this XCD
     for (;;) {
         if (tid < 64) {
-            // wave 0: ballots over groups of 64 buckets (lane = bucket), highest priority first
+            // wave 0, lane = priority bucket (highest first): the bucket's shared queue and the one homed on
+            // this XCD. Each scan reads the two (their counters lie in 256-byte blocks that only the workers
+            // of one XCD hammer) in one memory round trip; the other XCDs' queues only when both are empty.
             int64_t got = -1;
             const int64_t t_start = (int64_t)wall_clock64();
             for (;;) {
-                int fb = -1;  // queue to claim from
-                int sb = -1;  // first non-empty queue of any XCD (to steal from)
+                int cand[CLAIM_TRIES];  // queues to try, best first
+                int nc = 0;
                 bool any_open = false;
                 // a secondary leaves queues that the idle primaries can drain; nothing is taken while
                 // the other worker of this CU runs an item of the critical path
-                // every load of the scan is issued before any is used: one memory round trip for all groups
-                const int reserve = primary ? 0 : ld_relaxed(g.idle_primary);
+                // the lane index, opaque to the compiler here: addresses derived from it are formed per scan
+                // instead of being hoisted out of the worker loop, where they would hold registers through
+                // every item
+                int lane = tid;
+                asm volatile("" : "+v"(lane));
+                // (idle primaries of this XCD for its own queues, of all XCDs for shared queues and stealing)
+                int idle[NXCD];
+#pragma unroll
+                for (int x = 0; x < NXCD; x++) idle[x] = primary ? 0 : ld_relaxed(g.idle_primary + x * 64);
                 const bool blocked = ld_relaxed(g.cu_crit + my_key) != 0;
-                int cnt[MAX_QGROUPS];
+                const int cs = ld_relaxed(g.q_count + shared_queue(lane)), co = ld_relaxed(g.q_count + own_q + lane);
+                int reserve_all = 0, reserve_own = 0;
 #pragma unroll
-                for (int q = 0; q < MAX_QGROUPS; q++) {
-                    const int b = q * 64 + tid;
-                    cnt[q] = (q < ngroups && b < g.n_buckets) ? ld_relaxed(g.q_count + b) : INT_MIN;
+                for (int x = 0; x < NXCD; x++) {
+                    reserve_all += idle[x];
+                    if (x == my_xcd) reserve_own = idle[x];
                 }
-                // the candidates: the non-empty queues of the first group that has one for this XCD (else of
-                // the first group with any, to steal from), tried best first
-                unsigned long long cmask = 0, smask = 0;
-                int cq = -1, sq = -1;
+                const unsigned long long ms = __ballot(!blocked && cs > reserve_all);
+                const unsigned long long mo = __ballot(!blocked && co > reserve_own);
+                // within a bucket the shared queue (POTRF, TRSM, FP64 updates) comes first
+                for (unsigned long long m = ms | mo; m != 0 && nc < CLAIM_TRIES; m &= m - 1) {
+                    const int b = __ffsll((long long)m) - 1;
+                    if ((ms >> b) & 1) cand[nc++] = shared_queue(b);
+                    if (nc < CLAIM_TRIES && ((mo >> b) & 1)) cand[nc++] = own_q + b;
+                }
+                if (nc == 0 && !blocked) {  // nothing on this XCD: steal from the best bucket of another one
+                    int cx[NXCD];
 #pragma unroll
-                for (int q = 0; q < MAX_QGROUPS; q++) {  // groups of 64 queues, best first
-                    if (q >= ngroups || fb >= 0) break;
-                    const bool ne = !blocked && cnt[q] > reserve;
-                    const int home = s_qhome[(q * 64 + tid) % MAX_QUEUES];
-                    const bool mine = ne && (home < 0 || home == my_xcd);
-                    const unsigned long long mask = __ballot(mine), any = __ballot(ne);
-                    if (sb < 0 && any) {
-                        sb = q * 64 + __ffsll((long long)any) - 1;
-                        smask = any;
-                        sq = q;
+                    for (int x = 0; x < NXCD; x++) cx[x] = ld_relaxed(g.q_count + homed_queue(x, lane));
+                    unsigned long long mx[NXCD], any = 0;
+#pragma unroll
+                    for (int x = 0; x < NXCD; x++) {
+                        mx[x] = x == my_xcd ? 0ull : __ballot(cx[x] > reserve_all);
+                        any |= mx[x];
                     }
-                    if (mask) {
-                        fb = q * 64 + __ffsll((long long)mask) - 1;
-                        cmask = mask;
-                        cq = q;
+                    if (any) {
+                        const int b = __ffsll((long long)any) - 1;
+#pragma unroll
+                        for (int d = 1; d < NXCD; d++) {  // from the next XCD on, so that stealers spread
+                            const int x = (my_xcd + d) % NXCD;
+                            if (nc < CLAIM_TRIES && ((mx[x] >> b) & 1)) cand[nc++] = homed_queue(x, b);
+                        }
                     }
                 }
-                if (fb < 0 && sb < 0) {  // nothing claimable: is anything left to claim at all?
-                    int h[MAX_QGROUPS], size[MAX_QGROUPS];
+                if (nc == 0) {  // nothing claimable: is anything left to claim at all?
+                    int qs[NXCD + 1], hs[NXCD + 1];  // this lane's bucket: the shared queue, the homed ones
 #pragma unroll
-                    for (int q = 0; q < MAX_QGROUPS; q++) {
-                        const int b = q * 64 + tid;
-                        const bool ok = q < ngroups && b < g.n_buckets;
-                        h[q] = ok ? ld_relaxed(g.q_head + b) : 0;
-                        size[q] = ok ? g.q_base[b + 1] - g.q_base[b] : 0;
+                    for (int x = 0; x <= NXCD; x++) {
+                        const int q = x == NXCD ? shared_queue(lane) : homed_queue(x, lane);
+                        hs[x] = ld_relaxed(g.q_head + q);
+                        qs[x] = g.q_base[q + 1] - g.q_base[q];
                     }
+                    bool open = false;
 #pragma unroll
-                    for (int q = 0; q < MAX_QGROUPS; q++) any_open |= __ballot(h[q] < size[q]) != 0;
+                    for (int x = 0; x <= NXCD; x++) open |= hs[x] < qs[x];
+                    any_open = __ballot(open) != 0;
                 }
-                if (fb < 0 && sb >= 0) {  // nothing for this XCD: steal
-                    fb = sb;
-                    cmask = smask;
-                    cq = sq;
-                }
-                const bool all_claimed = fb < 0 && !any_open;
-                if (fb >= 0) {
+                const bool all_claimed = nc == 0 && !any_open;
+                if (nc > 0) {
                     int64_t e = -1;
                     if (tid == 0) {
-                        int slot = -1;
+                        int slot = -1, fb = -1;
                         // a credit lost to another worker: the next candidate at once (one atomic) rather than
                         // a new scan
-                        for (int attempt = 0; attempt < CLAIM_TRIES && cmask != 0; attempt++) {
-                            const int b = cq * 64 + __ffsll((long long)cmask) - 1;
-                            cmask &= cmask - 1;
+                        for (int c = 0; c < nc; c++) {
+                            const int b = cand[c];
+                            const int reserve = b >= own_q && b < own_q + QUEUE_BUCKETS ? reserve_own : reserve_all;
                             if (__hip_atomic_fetch_add(g.q_count + b, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >
                                 reserve) {
                                 slot = __hip_atomic_fetch_add(g.q_head + b, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -209,7 +219,7 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
             if (tid == 0) {
                 s_item = got;
                 if (primary && got >= 0)
-                    __hip_atomic_fetch_add(g.idle_primary, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_fetch_add(idle_primary, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
         __syncthreads();
@@ -278,7 +288,7 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
                     rec[5] = 0;
                 }
             }
-            if (primary) __hip_atomic_fetch_add(g.idle_primary, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (primary) __hip_atomic_fetch_add(idle_primary, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (t.crit) __hip_atomic_fetch_add(g.cu_crit + my_key, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
@@ -302,10 +312,10 @@ __global__ void k_graph_reset_a(DevGraph g, int n_slots) {
         g.cu_slots[x] = 0;
         g.cu_crit[x] = 0;
     }
+    for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < NXCD * 64; x += stride) g.idle_primary[x] = 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *g.done_items = 0;
         *g.abort_flag = 0;
-        *g.idle_primary = 0;
         if (g.trace_cursor) *g.trace_cursor = 0;
     }
 }
