@@ -773,6 +773,7 @@ static int graph_prepare(ttad_graph* g) {
         TTAD_HIP(g->d_deps0.alloc(n * 4));
         TTAD_HIP(g->d_deps.alloc(n * 4));
         TTAD_HIP(g->d_items_left.alloc(n * 4));
+        TTAD_HIP(g->d_gops.alloc((size_t)n * sizeof(GemmOperands)));
         TTAD_HIP(g->d_tail0.alloc(g->n_buckets * 4));
         TTAD_HIP(g->d_q_head.alloc(g->n_buckets * 4));
         TTAD_HIP(g->d_q_tail.alloc(g->n_buckets * 4));
@@ -845,6 +846,7 @@ static DevGraph graph_image(ttad_graph* g, bool trace, int64_t spin_limit) {
     d.init_slots = g->d_init_slot.as<int32_t>();
     d.deps = g->d_deps.as<int32_t>();
     d.items_left = g->d_items_left.as<int32_t>();
+    d.gops = g->d_gops.as<GemmOperands>();
     d.q_head = g->d_q_head.as<int32_t>();
     d.q_tail = g->d_q_tail.as<int32_t>();
     d.q_count = g->d_q_count.as<int32_t>();

@@ -7,6 +7,18 @@
 
 namespace ttad {
 
+// The operands of a GEMM/SYRK task, C(i,j) -= A(i,k) B(j,k)^T, resolved from the matrix's tile tables
+// before a run, so that an item finds them in one load with its task record instead of through the
+// matrix descriptor and its pointer tables (two more dependent memory round trips).
+struct alignas(16) GemmOperands {
+    const void* A;
+    const void* B;
+    void* C;
+    const void* PA;  // bf16 planes of A and B (null: none)
+    const void* PB;
+    int32_t m, n, K, pad;
+};
+
 // Device image of a task graph (built on the host from the Taskflow callbacks).
 struct DevGraph {
     int32_t n_tasks;
@@ -32,6 +44,7 @@ struct DevGraph {
     int32_t* done_items;
     int32_t* abort_flag;        // 1 = watchdog, 2 = queue overflow
     const DevMat* mats;
+    GemmOperands* gops;         // [n_tasks] resolved operands of the GEMM/SYRK tasks (filled before each run)
     uint64_t* trace;            // optional: TRACE_WORDS x u64 per item {task<<20|item, wg, t_start, t_end,
                                 // t_end of the successor release (the task's last item; else 0), 0}
     int32_t* trace_cursor;
@@ -170,7 +183,7 @@ __device__ __attribute__((noinline)) void trsm_call(const TS* L, long ldl, const
 // after spin_limit ticks or once *err is nonzero (and set *err = 1 when they time out).
 template <int VM>
 __device__ __forceinline__ bool exec_item(const DevTask& t, int it, const DevMat* mats, char* lds, int32_t* err,
-                                          int64_t spin_limit) {
+                                          int64_t spin_limit, const GemmOperands* go = nullptr) {
     if (t.kind == K_NOP) {  // spin task (scheduler-overhead benchmark): k device clock ticks, no tile access
         if (threadIdx.x == 0) {
             const uint64_t t0 = wall_clock64();
@@ -234,8 +247,32 @@ __device__ __forceinline__ bool exec_item(const DevTask& t, int it, const DevMat
         case K_SYRK:
         case K_GEMM: {
             const int k = t.k, i = t.i, j = t.j;
-            const int m = M.bsz[i], n = M.bsz[j], K = M.bsz[k];
             const bool syrk = (t.kind == K_SYRK);
+            const void *opA, *opB, *opPA, *opPB;
+            void* opC;
+            int m, n, K;
+            // resolved before the run (persistent executor); the FP64 instantiation keeps the table lookups,
+            // measured faster there (the FP64 items are long, the lookup latency is a small part of them)
+            constexpr bool use_go = (VM & (V_G32 | V_GBF16 | V_GBF16X3)) != 0;
+            if (use_go && go) {
+                opA = wave_uniform(go->A);
+                opB = wave_uniform(go->B);
+                opC = wave_uniform(go->C);
+                opPA = wave_uniform(go->PA);
+                opPB = wave_uniform(go->PB);
+                m = wave_uniform(go->m);
+                n = wave_uniform(go->n);
+                K = wave_uniform(go->K);
+            } else {
+                opA = tile(i, k);
+                opB = tile(j, k);
+                opC = tile(i, j);
+                opPA = planes(i, k);
+                opPB = planes(j, k);
+                m = M.bsz[i];
+                n = M.bsz[j];
+                K = M.bsz[k];
+            }
             int rb, cb;
             decode_gemm_item(it, (m + GEMM_BM - 1) / GEMM_BM, syrk, rb, cb);
             const int r0 = rb * GEMM_BM, c0 = cb * GEMM_BM;
@@ -249,13 +286,13 @@ __device__ __forceinline__ bool exec_item(const DevTask& t, int it, const DevMat
 #define TTAD_GEMM_ITEM(TS_, PREC_, CPRE_)                                                                         \
     if constexpr ((VM & V_ACC) != 0) {                                                                             \
         if (lock) {                                                                                                \
-            gemm_acc_call<TS_, PREC_, VM>((const TS_*)tile(i, k), m, (const TS_*)tile(j, k), n, (TS_*)tile(i, j), m, \
-                                          m, n, K, r0, c0, syrk, lock, err, spin_limit, lds);                      \
+            gemm_acc_call<TS_, PREC_, VM>((const TS_*)opA, m, (const TS_*)opB, n, (TS_*)opC, m, m, n, K, r0, c0,  \
+                                          syrk, lock, err, spin_limit, lds);                                       \
             return true;                                                                                           \
         }                                                                                                          \
     }                                                                                                              \
-    item_gemm<TS_, PREC_, CPRE_>((const TS_*)tile(i, k), m, (const TS_*)tile(j, k), n, (TS_*)tile(i, j), m, m, n,  \
-                                 K, r0, c0, syrk, -1.0, 1.0, lds, planes(i, k), planes(j, k));                     \
+    item_gemm<TS_, PREC_, CPRE_>((const TS_*)opA, m, (const TS_*)opB, n, (TS_*)opC, m, m, n, K, r0, c0, syrk,      \
+                                 -1.0, 1.0, lds, opPA, opPB);                                                      \
     return true;
             // Products read C after the last K-chunk: measured faster than reading it with the first chunk
             // for FP64 (factorization at n = 32768: 246.7 -> 229.4 ms) and for bf16x3 items (11 %).

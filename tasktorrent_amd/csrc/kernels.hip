@@ -231,13 +231,14 @@ __global__ __launch_bounds__(NTHREADS, vset_occupancy(VM)) void k_persistent(Dev
         const int task = __builtin_amdgcn_readfirstlane((int)(e >> 20));
         const int item = __builtin_amdgcn_readfirstlane((int)(e & 0xFFFFF));
         const DevTask t = g.tasks[task];
+        const GemmOperands go = g.gops[task];  // read with the task record (GEMM/SYRK items use it)
         if (tid == 0) {
             if (t.crit) __hip_atomic_fetch_add(g.cu_crit + my_key, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_fetch_add(g.xcd_run + (long)task * NXCD + my_xcd, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         uint64_t t0 = 0;
         if (g.trace && tid == 0) t0 = wall_clock64();
-        const bool ran = exec_item<VM>(t, item, g.mats, lds, g.abort_flag, g.spin_limit);
+        const bool ran = exec_item<VM>(t, item, g.mats, lds, g.abort_flag, g.spin_limit, &go);
         // Every wave waits until the L2 has acknowledged its own stores (gfx9: vmcnt counts stores too;
         // encoding vmcnt(0) expcnt(7) lgkmcnt(15)) before the barrier: after it, thread 0's decrement
         // below is issued only once all of this item's outputs sit in this XCD's L2.
@@ -301,6 +302,22 @@ __global__ void k_graph_reset_a(DevGraph g, int n_slots) {
     for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < g.n_tasks; x += stride) {
         g.deps[x] = g.deps0[x];
         g.items_left[x] = g.tasks[x].n_items;
+        const DevTask t = g.tasks[x];
+        if (t.kind == K_GEMM || t.kind == K_SYRK) {
+            const DevMat& M = g.mats[t.mat];
+            const long nt = M.nt;
+            GemmOperands o;
+            o.A = M.tiles[t.i + t.k * nt];
+            o.B = M.tiles[t.j + t.k * nt];
+            o.C = M.tiles[t.i + t.j * nt];
+            o.PA = M.planes ? M.planes[t.i + t.k * nt] : nullptr;
+            o.PB = M.planes ? M.planes[t.j + t.k * nt] : nullptr;
+            o.m = M.bsz[t.i];
+            o.n = M.bsz[t.j];
+            o.K = M.bsz[t.k];
+            o.pad = 0;
+            g.gops[x] = o;
+        }
     }
     for (long x = blockIdx.x * blockDim.x + threadIdx.x; x < (long)g.n_tasks * NXCD; x += stride) g.xcd_run[x] = 0;
     for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < g.n_buckets; x += stride) {

@@ -118,7 +118,7 @@ struct ttad_graph {
     bool with_planes = false;  // hand the matrices' bf16 planes to the items (a full Cholesky PTG only)
     std::vector<int32_t> spin_us;  // per task: spin time of a TTAD_SPIN task (microseconds)
     bool seeds_dirty = true, mats_dirty = true, running = false;
-    ttad::DevBuf d_tasks, d_out_ptr, d_out_idx, d_deps0, d_tail0, d_init, d_init_slot, d_deps, d_items_left, d_q_head, d_q_tail, d_q_count, d_xcd_run,
+    ttad::DevBuf d_tasks, d_out_ptr, d_out_idx, d_deps0, d_tail0, d_init, d_init_slot, d_deps, d_items_left, d_gops, d_q_head, d_q_tail, d_q_count, d_xcd_run,
         d_q_base, d_q_home, d_q_items, d_done, d_abort, d_mats, d_trace, d_trace_cursor, d_cu_slots;
     int n_init = 0;
     int workers = 0;
