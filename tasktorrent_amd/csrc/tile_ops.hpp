@@ -788,14 +788,27 @@ __device__ __forceinline__ d4_t mma16_reg(const double* P, d4_t T) {
     return out;
 }
 
+// 1/x to about one ulp without the FP64 division sequence: the hardware reciprocal estimate refined by two
+// Newton-Raphson steps r <- r + r (1 - x r), each of which doubles the number of correct bits. The POTRF
+// diagonal blocks divide by every pivot in turn, so the division's latency is on the factorization's
+// critical path.
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
 // X_KK = L_KK^{-1} for the 16x16 lower-triangular diagonal block at Lk (X block at Xk,
 // upper part left untouched). Row-by-row substitution with the partial sums in registers:
 // one barrier per row. All threads participate.
 __device__ __forceinline__ void tri_inv16(const double* Lk, double* Xk) {
     const int ti = threadIdx.x & 15, tc = threadIdx.x >> 4;
+    const double rd = rcp_nr(Lk[ti + ti * DP]);  // 1 / L(ti, ti), ahead of the row-by-row chain
     double acc = 0.0;
     for (int q = 0; q < 16; q++) {
-        if (ti == q && tc <= q) Xk[q + tc * DP] = ((q == tc ? 1.0 : 0.0) - acc) / Lk[q + q * DP];
+        if (ti == q && tc <= q) Xk[q + tc * DP] = ((q == tc ? 1.0 : 0.0) - acc) * rd;
         __syncthreads();
         if (ti > q && tc <= q) acc += Lk[ti + q * DP] * Xk[q + tc * DP];
     }
@@ -834,13 +847,13 @@ __device__ __forceinline__ int lds_potrf_inv64(double* D, double* X, int w) {
         for (int k = 0; k < 16; k++) {
             const double dkk = Dk[k + k * DP];
             if (tid == 0 && !(dkk > 0.0) && s_bad == 0 && k0 + k < w) s_bad = k0 + k + 1;
-            if (ti > k && tj > k && ti >= tj) Dk[ti + tj * DP] -= Dk[ti + k * DP] * Dk[tj + k * DP] / dkk;
+            if (ti > k && tj > k && ti >= tj) Dk[ti + tj * DP] -= Dk[ti + k * DP] * Dk[tj + k * DP] * rcp_nr(dkk);
             __syncthreads();
         }
         const double sj = sqrt(Dk[tj + tj * DP]);
         const double v = Dk[ti + tj * DP];
         __syncthreads();
-        if (ti > tj) Dk[ti + tj * DP] = v / sj;
+        if (ti > tj) Dk[ti + tj * DP] = v * rcp_nr(sj);
         else if (ti == tj) Dk[ti + tj * DP] = sj;
         __syncthreads();
         tri_inv16(Dk, X + k0 + k0 * DP);
