@@ -640,6 +640,7 @@ __device__ __forceinline__ void item_gemm(const TS* __restrict__ A, long lda, co
                         }
                         *reinterpret_cast<TTAD_GLOBAL f4u*>(base + 8 * j * ldu) = o;
                     }
+                    __syncthreads();  // the caller may use lds again
                     return;
                 }
             }
