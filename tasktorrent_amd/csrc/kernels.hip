@@ -1,5 +1,4 @@
 // HIP kernels of the tiled Cholesky library (gfx950).
-#include <climits>
 #include <algorithm>
 
 #include "kernels.hpp"
