@@ -125,8 +125,8 @@ static void launch_sweep(ttad_matrix* L, int forward, const double* R, long ldr,
                            ldd, X, ldx, nrhs, flags, ticket, err, spin);
 }
 
-// Lookahead sweeps (tile_sweep.hpp) for factors in FP32 tiles of at most 512 rows, 16 right-hand sides
-// at a time; the chain of 64-row blocks otherwise, and always with TTAD_SWEEP=block (for comparisons).
+// Lookahead sweeps (tile_sweep.hpp) for factors in FP32 tiles of at most 512 rows (at most LS_MAXT tiles),
+// 16 right-hand sides at a time; the chain of 64-row blocks otherwise, and always with TTAD_SWEEP=block (for comparisons).
 static bool lookahead_enabled() {
     static const bool on = [] {
         const char* v = getenv("TTAD_SWEEP");
@@ -135,7 +135,7 @@ static bool lookahead_enabled() {
     return on;
 }
 static bool lookahead_ok(const ttad_matrix* L) {
-    if (!lookahead_enabled() || L->storage != P_F32) return false;
+    if (!lookahead_enabled() || L->storage != P_F32 || L->nt > LS_MAXT) return false;
     for (int b : L->bsz)
         if (b > LS_BMAX || b <= 0) return false;
     return true;
@@ -228,7 +228,9 @@ hipError_t solve_init() {
     hipError_t e = hipFuncSetAttribute((const void*)k_tile_inv<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        TILE_INV_LDS);
     if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void*)k_lsweep, hipFuncAttributeMaxDynamicSharedMemorySize, LSWEEP_LDS);
+        e = hipFuncSetAttribute((const void*)k_lsweep<1>, hipFuncAttributeMaxDynamicSharedMemorySize, LSWEEP_LDS);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_lsweep<0>, hipFuncAttributeMaxDynamicSharedMemorySize, LSWEEP_LDS);
     if (e == hipSuccess)
         e = hipFuncSetAttribute((const void*)k_pair_gemm, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (gemm_item_lds<float, P_F32>()));
@@ -246,8 +248,12 @@ static void launch_lsweep(ttad_matrix* L, int forward, const double* R, long ldr
     float* const* tab = L->tab_tinv.as<float*>();
     const LSweepTab lt{P, nt, L->tab_blk_tile.as<int32_t>(), L->tab_blk_off.as<int32_t>(), L->tab_tfirst.as<int32_t>(),
                        tab, tab + nt, tab + 2 * nt, tab + 3 * nt, tab + 5 * nt, tab + 6 * nt};
-    hipLaunchKernelGGL(k_lsweep, dim3(P), dim3(NTHREADS), LSWEEP_LDS, s, L->desc.as<DevMat>(), lt, forward, R, ldr, Y,
-                       D, X, ldx, Rp, nrhs, flags, ticket, err, spin);
+    if (forward)
+        hipLaunchKernelGGL(k_lsweep<1>, dim3(P), dim3(NTHREADS), LSWEEP_LDS, s, L->desc.as<DevMat>(), lt, R, ldr, Y, D,
+                           X, ldx, Rp, nrhs, flags, ticket, err, spin);
+    else
+        hipLaunchKernelGGL(k_lsweep<0>, dim3(P), dim3(NTHREADS), LSWEEP_LDS, s, L->desc.as<DevMat>(), lt, R, ldr, Y, D,
+                           X, ldx, Rp, nrhs, flags, ticket, err, spin);
 }
 
 // X += (L L^T)^{-1} R in the work precision of L: lookahead sweeps (16 right-hand sides at a time) where they

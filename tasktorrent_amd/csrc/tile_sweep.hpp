@@ -280,55 +280,118 @@ __device__ __forceinline__ void lds_tile_mma(f4_t& acc, const float* As, const f
     acc += (c[0] + c[1]) + (c[2] + c[3]);
 }
 
-// A operand of wave w for block p x block q of the factor (as load_a, modes 0 and 1, with k = kperm(ks, lane / 16)),
-// from clamped addresses. Mode 1 (a transposed block, k contiguous in memory) loads four k-steps per 16 bytes.
-__device__ __forceinline__ void load_a_clamped(float (&a)[KS], const DevMat& M, const SweepTab& tab, int p, int q,
-                                               int mode) {
-    const int tp = uni(tab.blk_tile[p]), tq = uni(tab.blk_tile[q]);
-    const int op = uni(tab.blk_off[p]), oq = uni(tab.blk_off[q]);
-    const int wp = min(SB, uni(M.bsz[tp]) - op), wq = min(SB, uni(M.bsz[tq]) - oq);
+// B fragments of block c (64 k) of a tile-wide product from a swizzled row vector: the tile at row g0 with bu
+// rows; rows at or beyond bu read as 0. Four loads of 16 bytes per lane (k-steps 16 c .. 16 c + 15).
+__device__ __forceinline__ void load_rows_blk(float (&b)[KS], const float* Y, int g0, int bu, int c) {
+    const int lane = threadIdx.x & 63;
+    typedef float f4a_t __attribute__((ext_vector_type(4)));
+    const TTAD_GLOBAL f4a_t* p = reinterpret_cast<const TTAD_GLOBAL f4a_t*>(gmem(Y) + (long)g0 * LS_NR) + lane;
+    if (bu == LS_BMAX) {  // whole tile (uniform): no condition at all
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) {
+            const f4a_t v = p[(4 * c + jj) * 64];
+#pragma unroll
+            for (int q = 0; q < 4; q++) b[4 * jj + q] = v[q];
+        }
+        return;
+    }
+    const int kl = lane >> 4;
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {  // the buffer is padded past the last tile: these loads stay in it
+        const int j = 4 * c + jj;
+        const f4a_t v = p[j * 64];
+#pragma unroll
+        for (int q = 0; q < 4; q++) b[4 * jj + q] = 16 * j + 4 * kl + q < bu ? v[q] : 0.0f;
+    }
+}
+
+// The far-tile stream's loads: whole 64 x 64 blocks and whole 64-row blocks of the right-hand sides, with no
+// condition at all, so that the stream is straight-line code and each product waits for its own slot only.
+// Row m of a mode-0 block is clamped by the caller (rows beyond the workgroup's block are computed and dropped).
+__device__ __forceinline__ void load_a_whole(float (&a)[KS], const float* Tp, long ld, int row, int oq, int mode) {
+    const int kg = (threadIdx.x & 63) >> 4;
+    const TTAD_GLOBAL float* T = gmem(Tp);
+    if (mode == 0) {
+        const TTAD_GLOBAL float* P = T + row + (long)oq * ld;
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) a[ks] = P[(long)kperm(ks, kg) * ld];
+    } else {
+        typedef float f4u_t __attribute__((ext_vector_type(4), aligned(4)));
+        const TTAD_GLOBAL float* P = T + oq + (long)row * ld;
+#pragma unroll
+        for (int j = 0; j < KS / 4; j++) {
+            const f4u_t v = *reinterpret_cast<const TTAD_GLOBAL f4u_t*>(P + 16 * j + 4 * kg);
+#pragma unroll
+            for (int q = 0; q < 4; q++) a[4 * j + q] = v[q];
+        }
+    }
+}
+__device__ __forceinline__ void load_rows_whole(float (&b)[KS], const float* Y, int g0, int c) {
+    typedef float f4a_t __attribute__((ext_vector_type(4)));
+    const TTAD_GLOBAL f4a_t* p =
+        reinterpret_cast<const TTAD_GLOBAL f4a_t*>(gmem(Y) + (long)g0 * LS_NR) + (threadIdx.x & 63);
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+        const f4a_t v = p[(4 * c + jj) * 64];
+#pragma unroll
+        for (int q = 0; q < 4; q++) b[4 * jj + q] = v[q];
+    }
+}
+
+// A fragments of wave w for one 64 x 64 block of a factor tile T (column-major, ld) with k = kperm(ks, lane / 16):
+// mode 0 reads rows op + m and columns oq + k, mode 1 the transposed block (rows oq + k, columns op + m; k
+// contiguous in memory, four k-steps per 16-byte load). Zero beyond wp rows (m) and wq columns (k), from clamped
+// addresses.
+__device__ __forceinline__ void load_a_blk(float (&a)[KS], const float* Tp, long ld, int op, int oq, int wp, int wq,
+                                           int mode) {
     const int lane = threadIdx.x & 63;
     const int m = (threadIdx.x >> 6) * 16 + (lane & 15), kg = lane >> 4;
-    const int mc = min(m, wp - 1);
-    if (wp == SB && wq == SB) {  // whole 64 x 64 block (uniform): no condition at all
+    const TTAD_GLOBAL float* T = gmem(Tp);
+    if (wp == SB && wq == SB) {  // whole block (uniform): no condition at all
         if (mode == 0) {
-            const long ld = M.bsz[tp];
-            const TTAD_GLOBAL float* T = gmem((const float*)M.tiles[tp + (long)tq * M.nt]) + (op + m) + (long)oq * ld;
+            const TTAD_GLOBAL float* P = T + (op + m) + (long)oq * ld;
 #pragma unroll
-            for (int ks = 0; ks < KS; ks++) a[ks] = T[(long)kperm(ks, kg) * ld];
+            for (int ks = 0; ks < KS; ks++) a[ks] = P[(long)kperm(ks, kg) * ld];
         } else {
             typedef float f4u_t __attribute__((ext_vector_type(4), aligned(4)));
-            const long ld = M.bsz[tq];
-            const TTAD_GLOBAL float* T = gmem((const float*)M.tiles[tq + (long)tp * M.nt]) + oq + (long)(op + m) * ld;
+            const TTAD_GLOBAL float* P = T + oq + (long)(op + m) * ld;
 #pragma unroll
             for (int j = 0; j < KS / 4; j++) {
-                const f4u_t v = *reinterpret_cast<const TTAD_GLOBAL f4u_t*>(T + 16 * j + 4 * kg);
+                const f4u_t v = *reinterpret_cast<const TTAD_GLOBAL f4u_t*>(P + 16 * j + 4 * kg);
 #pragma unroll
                 for (int q = 0; q < 4; q++) a[4 * j + q] = v[q];
             }
         }
         return;
     }
+    const int mc = min(m, wp - 1);
     if (mode == 0) {
-        const long ld = M.bsz[tp];
-        const TTAD_GLOBAL float* T = gmem((const float*)M.tiles[tp + (long)tq * M.nt]) + (op + mc) + (long)oq * ld;
+        const TTAD_GLOBAL float* P = T + (op + mc) + (long)oq * ld;
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
             const int k = kperm(ks, kg);
-            const float v = T[(long)min(k, wq - 1) * ld];
+            const float v = P[(long)min(k, wq - 1) * ld];
             a[ks] = (m < wp && k < wq) ? v : 0.0f;
         }
     } else {
-        const long ld = M.bsz[tq];
-        const TTAD_GLOBAL float* T = gmem((const float*)M.tiles[tq + (long)tp * M.nt]) + oq + (long)(op + mc) * ld;
+        const TTAD_GLOBAL float* P = T + oq + (long)(op + mc) * ld;
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
             const int k = kperm(ks, kg);
-            const float v = T[min(k, wq - 1)];
+            const float v = P[min(k, wq - 1)];
             a[ks] = (m < wp && k < wq) ? v : 0.0f;
         }
     }
 }
+
+// A far tile of one workgroup, looked up once per sweep: its factor tile (L_tu forward, L_ut backward), its
+// rows and its first row.
+struct FarTile {
+    const float* T;
+    int32_t rows;
+    int32_t row0;
+};
+constexpr int LS_MAXT = 1024;  // most tiles of a lookahead sweep (the far-tile table: 16 KB of LDS)
 
 // Waits (thread 0 polls, the barrier passes it on) until *cnt >= want.
 __device__ __forceinline__ void ls_wait(const int32_t* cnt, int want, int32_t* err, int64_t spin_limit, bool slow) {
@@ -338,10 +401,24 @@ __device__ __forceinline__ void ls_wait(const int32_t* cnt, int want, int32_t* e
 
 // One 64-row block p of a lookahead sweep (see the top of this file). flags[t]: blocks of tile t whose r'
 // is published; flags[nt + t]: blocks of tile t whose output (y or x) is final.
-__device__ __forceinline__ void item_lsweep(const DevMat& M, const LSweepTab& lt, int forward, int p, const double* R,
-                                            long ldr, float* Y, float* D, double* X, long ldx, float* Rp, int nrhs,
-                                            int32_t* flags, int32_t* err, int64_t spin_limit, float* As) {
-    const SweepTab tab{lt.P, lt.blk_tile, lt.blk_off};
+//
+// The far products stream 64 x 64 blocks of the factor and 64-row blocks of the right-hand sides through a
+// ring of LS_RING register slots: the loads of stream block i + LS_RING go out right after the products of
+// block i, so LS_RING - 1 blocks are in flight behind the products, across far tiles too, and no load in the
+// stream depends on a condition (a wait before a product covers that product's slot only). Each wave polls the
+// far tiles' counters itself, up to 64 tiles at a time, and acquires once per batch of final tiles. The last
+// far tile (the one the chain waits for) comes after the stream, with its factor blocks loaded before its wait.
+// All sums run in the same order as a plain loop over far tiles, blocks and k-steps, so the results do not
+// depend on timing.
+constexpr int LS_RING = 4;
+template <int FWD>
+__device__ __forceinline__ void item_lsweep(const DevMat& M, const LSweepTab& lt, int p, const double* R, long ldr,
+                                            float* Y, float* D, double* X, long ldx, float* Rp, int nrhs,
+                                            int32_t* flags, int32_t* err, int64_t spin_limit, FarTile* ft, float* As) {
+    constexpr bool forward = FWD != 0;
+    constexpr int mode = forward ? 0 : 1;
+    constexpr int NBT = LS_BMAX / SB;
+    static_assert(NBT % LS_RING == 0, "a tile's blocks fill the ring a whole number of times");
     const int nt = lt.nt;
     const int t = uni(lt.blk_tile[p]), boff_p = uni(lt.blk_off[p]);
     const int bt = uni(M.bsz[t]), nbt = uni(lt.tfirst[t + 1]) - uni(lt.tfirst[t]);
@@ -365,37 +442,110 @@ __device__ __forceinline__ void item_lsweep(const DevMat& M, const LSweepTab& lt
     const int btn = has_near ? uni(M.bsz[tn]) : 0, nbn = has_near ? uni(lt.tfirst[tn + 1]) - uni(lt.tfirst[tn]) : 0;
     if (has_near) stage_rows(As, (forward ? lt.mfac[t] : lt.kfac[t]) + boff_p, bt, w, btn);
     f4_t acc = {0, 0, 0, 0};
-    float b[LS_KS];
+    float ra[LS_RING][KS], rb[LS_RING][KS];  // the ring: factor blocks and right-hand-side blocks
 
-    // A. far tiles (t - 2 and before in sweep order), each once all its outputs are final
+    // A. far tiles (t - 2 and before in sweep order), each once all its outputs are final. Their geometry goes
+    // to LDS first, so that the stream reads no table from global memory (such a load's wait would drain it).
     const int nfar = forward ? max(0, t - 1) : max(0, nt - t - 2);
-    const int mode = forward ? 0 : 1;
-    // The matrix operand of a far tile (its 8 blocks, 16 values per lane each) is loaded before waiting for the
-    // tile, the next tile's right after the current products: only the right-hand sides load after a wait.
-    constexpr int NBT = LS_BMAX / SB;
-    float a[NBT][KS];
-    auto load_tile_a = [&](int u) {
-        const int q0 = uni(lt.tfirst[u]), nbu = uni(lt.tfirst[u + 1]) - q0;
-#pragma unroll
-        for (int c = 0; c < NBT; c++)
-            if (c < nbu) load_a_clamped(a[c], M, tab, p, q0 + c, mode);
+    auto far_u = [&](int f) { return forward ? f : nt - 1 - f; };
+    for (int f = threadIdx.x; f < nfar; f += NTHREADS) {
+        const int u = far_u(f);
+        ft[f] = FarTile{(const float*)M.tiles[forward ? t + (long)u * nt : u + (long)t * nt], M.bsz[u], M.boff[u]};
+    }
+    __syncthreads();
+    int ready = 0;  // far tiles f < ready are final and acquired by this wave
+    auto make_ready = [&](int f) {
+        if (f < ready) return;
+        const int fi = f + lane;
+        bool fin = false;
+        if (fi < nfar)
+            fin = __hip_atomic_load(ocnt + far_u(fi), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >=
+                  (ft[fi].rows + SB - 1) / SB;
+        const unsigned long long notfin = ~(unsigned long long)__ballot(fin);
+        const int run = notfin == 0 ? 64 : (int)__builtin_ctzll(notfin);  // final tiles from f on, in order
+        if (run > 0) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            ready = f + run;
+        } else {
+            (void)wait_count(ocnt + far_u(f), (uni(ft[f].rows) + SB - 1) / SB, err, spin_limit, f + 1 < nfar);
+            ready = f + 1;
+        }
     };
-    if (nfar > 0) load_tile_a(forward ? 0 : nt - 1);
-    for (int f = 0; f < nfar; f++) {
-        const int u = forward ? f : nt - 1 - f;
-        const int nbu = uni(lt.tfirst[u + 1]) - uni(lt.tfirst[u]);
-        ls_wait(ocnt + u, nbu, err, spin_limit, f + 1 < nfar);
-        load_rows16(b, Src, uni(M.boff[u]), uni(M.bsz[u]));
+    const int arow = boff_p + min(wv * 16 + (lane & 15), w - 1);  // this lane's row of the workgroup's block
+    // Stream block (far tile f, block c). Past the end of a tile or of the stream it loads a block of the stream
+    // again (the products skip it): every slot is refilled by the same loads on every path, so that the wait
+    // before a product covers that slot's loads only.
+    // The stream covers far tiles f0 .. f_end - 1; the last far tile (t - 2 in sweep order, the one on the chain)
+    // comes after it, with its factor blocks loaded before its wait.
+    const int f_end = nfar - 1;
+    auto issue = [&](float (&a)[KS], float (&b)[KS], int f, int c) {
+        const int fc = min(f, f_end - 1);
+        const int rows = uni(ft[fc].rows);
+        const int cc = min(c, rows / SB - 1);
+        load_a_whole(a, wave_uniform(ft[fc].T), forward ? bt : rows, arow, SB * cc, mode);
+        load_rows_whole(b, Src, uni(ft[fc].row0), cc);
+    };
+    int f0 = 0;
+    if (nfar > 0 && (uni(ft[0].rows) & (SB - 1)) != 0) {
+        // the last tile of the matrix with a partial last block (the first far tile of a backward sweep): block
+        // by block, zero beyond its rows, before the stream
+        make_ready(0);
+        const int rows = uni(ft[0].rows), row0 = uni(ft[0].row0);
+        const float* T = wave_uniform(ft[0].T);
+        for (int c = 0; c * SB < rows; c++) {
+            load_a_blk(ra[0], T, forward ? bt : rows, boff_p, SB * c, w, min(SB, rows - SB * c), mode);
+            load_rows_blk(rb[0], Src, row0, rows, c);
+#pragma unroll
+            for (int k = 0; k < KS; k++) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ra[0][k], rb[0][k], acc, 0, 0, 0);
+        }
+        f0 = 1;
+    }
+    if (f0 < f_end) {
+        make_ready(f0);
+#pragma unroll
+        for (int c = 0; c < LS_RING; c++) issue(ra[c], rb[c], f0, c);
+    }
+    for (int f = f0; f < f_end; f++) {
+        const int rows = uni(ft[f].rows);
+#pragma unroll
+        for (int c = 0; c < NBT; c++) {
+            if (c * SB < rows) {
+#pragma unroll
+                for (int k = 0; k < KS; k++)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ra[c % LS_RING][k], rb[c % LS_RING][k], acc, 0, 0, 0);
+            }
+            if (c + LS_RING < NBT) {
+                issue(ra[c % LS_RING], rb[c % LS_RING], f, c + LS_RING);
+            } else {
+                if (c + LS_RING == NBT && f + 1 < f_end) make_ready(f + 1);
+                issue(ra[c % LS_RING], rb[c % LS_RING], f + 1, c + LS_RING - NBT);
+            }
+        }
+    }
+    if (f_end >= f0) {  // a whole tile (a partial one is the first far tile, done above)
+        const int rows = uni(ft[f_end].rows), row0 = uni(ft[f_end].row0);
+        const float* T = wave_uniform(ft[f_end].T);
+        float al[NBT][KS];
 #pragma unroll
         for (int c = 0; c < NBT; c++)
-            if (c < nbu) {
+            if (c * SB < rows) load_a_whole(al[c], T, forward ? bt : rows, arow, SB * c, mode);
+        make_ready(f_end);
 #pragma unroll
-                for (int k = 0; k < KS; k++) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[c][k], b[c * KS + k], acc, 0, 0, 0);
+        for (int c = 0; c < LS_RING; c++)
+            if (c * SB < rows) load_rows_whole(rb[c], Src, row0, c);
+#pragma unroll
+        for (int c = 0; c < NBT; c++) {
+            if (c * SB < rows) {
+#pragma unroll
+                for (int k = 0; k < KS; k++)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(al[c][k], rb[c % LS_RING][k], acc, 0, 0, 0);
             }
-        if (f + 1 < nfar) load_tile_a(forward ? f + 1 : nt - 2 - f);
+            if (c + LS_RING < NBT && (c + LS_RING) * SB < rows) load_rows_whole(rb[c % LS_RING], Src, row0, c + LS_RING);
+        }
     }
     // B. the neighbour tile through its published r' (the chain step)
     if (has_near) {
+        float b[LS_KS];
         ls_wait(rcnt + tn, nbn, err, spin_limit, false);  // also publishes As
         load_rows16(b, Rp, uni(M.boff[tn]), btn);
         lds_tile_mma(acc, As, b);
@@ -411,6 +561,7 @@ __device__ __forceinline__ void item_lsweep(const DevMat& M, const LSweepTab& lt
     if (threadIdx.x == 0) __hip_atomic_fetch_add(rcnt + t, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     // C. output rows: Tinv_t[s, :] r'_t (forward) or U_t[s, :] r''_t (backward). The operand is stored in fragment
     // order (k_tile_inv): 32 loads of 16 bytes per lane, issued before the wait for the rest of r'_t.
+    float at[NBT][KS];
     {
         typedef float f4a_t __attribute__((ext_vector_type(4)));
         const TTAD_GLOBAL f4a_t* T = reinterpret_cast<const TTAD_GLOBAL f4a_t*>(gmem(forward ? lt.tfrag[t] : lt.ufrag[t])) +
@@ -419,16 +570,20 @@ __device__ __forceinline__ void item_lsweep(const DevMat& M, const LSweepTab& lt
         for (int j = 0; j < LS_KS / 4; j++) {
             const f4a_t v = T[j * 64];
 #pragma unroll
-            for (int q = 0; q < 4; q++) a[j / 4][(j % 4) * 4 + q] = v[q];
+            for (int q = 0; q < 4; q++) at[j / 4][(j % 4) * 4 + q] = v[q];
         }
     }
     ls_wait(rcnt + t, nbt, err, spin_limit, false);
-    load_rows16(b, Rp, tg0, bt);
+#pragma unroll
+    for (int c = 0; c < LS_RING; c++) load_rows_blk(rb[c], Rp, tg0, bt, c);
     acc = f4_t{0, 0, 0, 0};
 #pragma unroll
-    for (int c = 0; c < NBT; c++)
+    for (int c = 0; c < NBT; c++) {
 #pragma unroll
-        for (int k = 0; k < KS; k++) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[c][k], b[c * KS + k], acc, 0, 0, 0);
+        for (int k = 0; k < KS; k++)
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(at[c][k], rb[c % LS_RING][k], acc, 0, 0, 0);
+        if (c + LS_RING < NBT) load_rows_blk(rb[c % LS_RING], Rp, tg0, bt, c + LS_RING);
+    }
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         const int m = wv * 16 + Mfma<float>::row(lane, r), n = lane & 15;
@@ -449,10 +604,11 @@ __device__ __forceinline__ void item_lsweep(const DevMat& M, const LSweepTab& lt
 
 // One workgroup per 64-row block, claimed through a ticket counter in sweep order: every block a workgroup
 // waits on (earlier tiles, and the blocks of its own tile) holds a smaller ticket or belongs to its own tile,
-// whose blocks hold consecutive tickets, so it is running or done.
-__global__ __launch_bounds__(NTHREADS) void k_lsweep(const DevMat* Mp, LSweepTab tab, int forward, const double* R,
-                                                     long ldr, float* Y, float* D, double* X, long ldx, float* Rp,
-                                                     int nrhs, int32_t* flags, int32_t* ticket, int32_t* err,
+// whose blocks hold consecutive tickets, so it is running or done. FWD: forward sweep (else backward).
+template <int FWD>
+__global__ __launch_bounds__(NTHREADS) void k_lsweep(const DevMat* Mp, LSweepTab tab, const double* R, long ldr,
+                                                     float* Y, float* D, double* X, long ldx, float* Rp, int nrhs,
+                                                     int32_t* flags, int32_t* ticket, int32_t* err,
                                                      int64_t spin_limit) {
     extern __shared__ __align__(16) char lds[];
     __shared__ int s_t;
@@ -460,9 +616,10 @@ __global__ __launch_bounds__(NTHREADS) void k_lsweep(const DevMat* Mp, LSweepTab
     __syncthreads();
     const int q = s_t;
     if (q >= tab.P) return;
-    const int p = forward ? q : tab.P - 1 - q;
-    item_lsweep(*Mp, tab, forward, p, R, ldr, Y, D, X, ldx, Rp, nrhs, flags, err, spin_limit,
-                reinterpret_cast<float*>(lds));
+    const int p = FWD ? q : tab.P - 1 - q;
+    __shared__ FarTile ft[LS_MAXT];
+    item_lsweep<FWD>(*Mp, tab, p, R, ldr, Y, D, X, ldx, Rp, nrhs, flags, err, spin_limit, ft,
+                     reinterpret_cast<float*>(lds));
 }
 
 }  // namespace ttad

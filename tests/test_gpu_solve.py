@@ -2,6 +2,8 @@
 
 - ttad_solve with the FP64 factor against the oracle's two triangular solves
   (2d_cholesky.cpp:676-681), relative 1e-12 on well-conditioned A;
+- ttad_solve with an FP32 factor (the tile-chained lookahead sweeps) against the host's
+  triangular solves with the same factor, and bitwise repeatable;
 - ttad_residual against b - A x computed on the host in FP64 (1e-13 relative) and
   the normwise backward error definition;
 - mixed precision + iterative refinement reaches eta <= 1e-12 (the bar of the
@@ -49,6 +51,36 @@ def test_solve_fp64_factor(ctx, oracle, n, b, nrhs):
     Xref = np.linalg.solve(A, B)
     assert np.linalg.norm(X - Xref) / np.linalg.norm(Xref) < 1e-12
     assert backward_error(A, X, B) < 1e-15
+
+
+@pytest.mark.parametrize("sizes,nrhs", [([512] * 6, 16), ([512, 320, 448, 192, 384, 64, 100], 20),
+                                        ([128] * 9, 3), ([512] * 3, 1)])
+def test_lookahead_sweeps_fp32_factor(ctx, oracle, sizes, nrhs):
+    """ttad_solve with an FP32 factor whose tiles have at most 512 rows (the tile-chained lookahead
+    sweeps) against the two triangular solves on the host with the same factor in FP64. The sweeps
+    multiply by explicit inverses of the diagonal tiles in FP32, so on A = B B^T/n + I (condition
+    number below 5) the solution agrees to a small multiple of FP32's unit roundoff times the tile
+    size. Tiles of different sizes (multiples of 64 but the last, as the solve requires) exercise
+    short far tiles and a partial last block; nrhs = 20 takes two 16-column passes; three tiles
+    leave a single far tile. Two calls give the same bits."""
+    from scipy.linalg import solve_triangular
+    n = sum(sizes)
+    A = oracle.spd_bbt(n, n + nrhs)
+    W = tt.TiledMatrix(ctx, sizes, "f32")
+    W.set(A)
+    assert ctx.potrf_tiled(W, tt.TTAD_SCHED_PERSISTENT, 2, "f32") == 0
+    L = W.get()
+    B = np.asfortranarray(np.random.default_rng(nrhs + n).standard_normal((n, nrhs)))
+    dB = _dev_cols(B)
+    dX1, dX2 = _dev_cols(np.zeros((n, nrhs))), _dev_cols(np.zeros((n, nrhs)))
+    ctx.solve(W, nrhs, dB, dX1)
+    ctx.solve(W, nrhs, dB, dX2)
+    X1, X2 = from_dev(dX1), from_dev(dX2)
+    W.close()
+    Xref = solve_triangular(L.T, solve_triangular(L, B, lower=True), lower=False)
+    assert np.array_equal(X1, X2)
+    err = np.linalg.norm(X1 - Xref) / np.linalg.norm(Xref)
+    assert err < 2e-5, err
 
 
 @pytest.mark.parametrize("nrhs", [1, 16, 40])
