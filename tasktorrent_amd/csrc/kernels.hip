@@ -422,35 +422,39 @@ __global__ void k_unpack(double* __restrict__ dst, long ldd, const DevMat* M, in
         dst[(ri + p) + (long)(cj + q) * ldd] = (double)t[e];
     }
 }
-// Tile-wise copy with conversion between two matrices of the same partition (one workgroup of
-// 256 per lower tile). Element pairs move as one vector access, four pairs in flight per thread;
-// a tile whose pointers or element count do not allow pairs is copied element by element.
+// Tile-wise copy with conversion between two matrices of the same partition, one workgroup of 256 per lower
+// tile (workgroup x -> tile (i, j), x = i (i + 1) / 2 + j). Four consecutive elements move as one vector access
+// (two 16-byte loads of FP64, one 16-byte store of FP32), four quads in flight per thread; a tile whose pointers
+// or element count do not allow quads is copied element by element.
 template <typename TD, typename TSRC>
 __global__ __launch_bounds__(256) void k_copy_tiles(const DevMat* S, const DevMat* D, int nt) {
-    typedef TSRC s2 __attribute__((ext_vector_type(2)));
-    typedef TD d2 __attribute__((ext_vector_type(2)));
-    const int i = blockIdx.x, j = blockIdx.y;
-    if (i < j) return;
+    typedef TSRC s4 __attribute__((ext_vector_type(4)));
+    typedef TD d4 __attribute__((ext_vector_type(4)));
+    const int x = blockIdx.x;
+    int i = (int)((sqrt(8.0 * x + 1.0) - 1.0) * 0.5);
+    while ((i + 1) * (i + 2) / 2 <= x) i++;
+    while (i * (i + 1) / 2 > x) i--;
+    const int j = x - i * (i + 1) / 2;
     const TSRC* s = (const TSRC*)S->tiles[i + (long)j * nt];
     TD* d = (TD*)D->tiles[i + (long)j * nt];
     const long cnt = (long)S->bsz[i] * S->bsz[j];
-    const bool pairs = (cnt % 2) == 0 && ((uintptr_t)s % sizeof(s2)) == 0 && ((uintptr_t)d % sizeof(d2)) == 0;
-    if (!pairs) {
+    const bool quads = (cnt % 4) == 0 && ((uintptr_t)s % sizeof(s4)) == 0 && ((uintptr_t)d % sizeof(d4)) == 0;
+    if (!quads) {
         for (long e = threadIdx.x; e < cnt; e += 256) d[e] = (TD)s[e];
         return;
     }
-    const TTAD_GLOBAL s2* sp = gmem((const s2*)s);
-    TTAD_GLOBAL d2* dp = gmem((d2*)d);
-    const long np = cnt / 2;
+    const TTAD_GLOBAL s4* sp = gmem((const s4*)s);
+    TTAD_GLOBAL d4* dp = gmem((d4*)d);
+    const long nq = cnt / 4;
     long e = threadIdx.x;
-    for (; e + 3 * 256 < np; e += 4 * 256) {
-        const s2 v0 = sp[e], v1 = sp[e + 256], v2 = sp[e + 512], v3 = sp[e + 768];
-        dp[e] = __builtin_convertvector(v0, d2);
-        dp[e + 256] = __builtin_convertvector(v1, d2);
-        dp[e + 512] = __builtin_convertvector(v2, d2);
-        dp[e + 768] = __builtin_convertvector(v3, d2);
+    for (; e + 3 * 256 < nq; e += 4 * 256) {
+        const s4 v0 = sp[e], v1 = sp[e + 256], v2 = sp[e + 512], v3 = sp[e + 768];
+        dp[e] = __builtin_convertvector(v0, d4);
+        dp[e + 256] = __builtin_convertvector(v1, d4);
+        dp[e + 512] = __builtin_convertvector(v2, d4);
+        dp[e + 768] = __builtin_convertvector(v3, d4);
     }
-    for (; e < np; e += 256) dp[e] = __builtin_convertvector(sp[e], d2);
+    for (; e < nq; e += 256) dp[e] = __builtin_convertvector(sp[e], d4);
 }
 
 // ---------------------------------------------------------------------------
@@ -618,13 +622,14 @@ void launch_unpack(int storage, double* dst, long ldd, const DevMat* M, int nt, 
 }
 
 void launch_copy_tiles(int dst_storage, int src_storage, const DevMat* S, const DevMat* D, int nt, hipStream_t s) {
+    const unsigned nlow = (unsigned)nt * (nt + 1) / 2;
     if (dst_storage == P_F32 && src_storage == P_F64)
-        hipLaunchKernelGGL((k_copy_tiles<float, double>), dim3(nt, nt), dim3(256), 0, s, S, D, nt);
+        hipLaunchKernelGGL((k_copy_tiles<float, double>), dim3(nlow), dim3(256), 0, s, S, D, nt);
     else if (dst_storage == P_F64 && src_storage == P_F32)
-        hipLaunchKernelGGL((k_copy_tiles<double, float>), dim3(nt, nt), dim3(256), 0, s, S, D, nt);
+        hipLaunchKernelGGL((k_copy_tiles<double, float>), dim3(nlow), dim3(256), 0, s, S, D, nt);
     else if (dst_storage == P_F64)
-        hipLaunchKernelGGL((k_copy_tiles<double, double>), dim3(nt, nt), dim3(256), 0, s, S, D, nt);
-    else hipLaunchKernelGGL((k_copy_tiles<float, float>), dim3(nt, nt), dim3(256), 0, s, S, D, nt);
+        hipLaunchKernelGGL((k_copy_tiles<double, double>), dim3(nlow), dim3(256), 0, s, S, D, nt);
+    else hipLaunchKernelGGL((k_copy_tiles<float, float>), dim3(nlow), dim3(256), 0, s, S, D, nt);
 }
 
 }  // namespace ttad

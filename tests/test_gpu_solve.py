@@ -268,7 +268,7 @@ def test_solve_ir_zero_rhs_and_padded_ld(ctx, oracle):
 
 @pytest.mark.parametrize("dst", ["f32", "f64"])
 def test_matrix_copy_ragged_tiles(ctx, dst):
-    # tiles of 130, 67 and 3 rows: element counts both even (copied in pairs) and odd (one by one);
+    # tiles of 130, 67 and 3 rows: element counts divisible by four (copied in quads) and not (one by one);
     # FP64 -> FP32 rounds to nearest like numpy's astype
     rng = np.random.default_rng(3)
     sizes = [130, 67, 3]
