@@ -249,10 +249,10 @@ static void launch_lsweep(ttad_matrix* L, int forward, const double* R, long ldr
     const LSweepTab lt{P, nt, L->tab_blk_tile.as<int32_t>(), L->tab_blk_off.as<int32_t>(), L->tab_tfirst.as<int32_t>(),
                        tab, tab + nt, tab + 2 * nt, tab + 3 * nt, tab + 5 * nt, tab + 6 * nt};
     if (forward)
-        hipLaunchKernelGGL(k_lsweep<1>, dim3(P), dim3(NTHREADS), LSWEEP_LDS, s, L->desc.as<DevMat>(), lt, R, ldr, Y, D,
+        hipLaunchKernelGGL(k_lsweep<1>, dim3(P), dim3(LS_THREADS), LSWEEP_LDS, s, L->desc.as<DevMat>(), lt, R, ldr, Y, D,
                            X, ldx, Rp, nrhs, flags, ticket, err, spin);
     else
-        hipLaunchKernelGGL(k_lsweep<0>, dim3(P), dim3(NTHREADS), LSWEEP_LDS, s, L->desc.as<DevMat>(), lt, R, ldr, Y, D,
+        hipLaunchKernelGGL(k_lsweep<0>, dim3(P), dim3(LS_THREADS), LSWEEP_LDS, s, L->desc.as<DevMat>(), lt, R, ldr, Y, D,
                            X, ldx, Rp, nrhs, flags, ticket, err, spin);
 }
 

@@ -11,7 +11,8 @@
 //   A. acc = sum_{u <= t-2} L_tu[s, :] y_u, each tile u as soon as all its blocks of y are final;
 //   B. acc += M_t[s, :] r'_{t-1} once every block of r'_{t-1} is published (M_t[s, :] staged in LDS ahead);
 //   2. r'_t[s] = r_t[s] - acc is published (tile counter of r');
-//   C. once all of r'_t is published, y_t[s] = Tinv_t[s, :] r'_t (Tinv_t[s, :] staged in LDS meanwhile).
+//   C. once all of r'_t is published, y_t[s] = Tinv_t[s, :] r'_t (Tinv_t[s, :] loaded at the start, by the
+//      workgroup's other four waves).
 // So the chain r'_{t-1} -> r'_t costs one counter round trip and one 64 x b x 16 product; y_t follows off
 // the chain. The backward sweep L^T x = y runs the same way from the last tile, with U_t = Tinv_t^T,
 // KT_t = (Tinv_{t+1} L_{t+1,t})^T, x_t = U_t r''_t and X += x.
@@ -410,7 +411,15 @@ __device__ __forceinline__ void ls_wait(const int32_t* cnt, int want, int32_t* e
 // far tile (the one the chain waits for) comes after the stream, with its factor blocks loaded before its wait.
 // All sums run in the same order as a plain loop over far tiles, blocks and k-steps, so the results do not
 // depend on timing.
+//
+// The workgroup has eight waves. Waves 0-3 run steps A, B and 2, then wait for all of r'_t (thread 0 polls
+// with no load of its own outstanding) and end; waves 4-7 load step C's operand (Tinv_t or U_t rows, 128 KB)
+// at the start, pass the same barriers and run step C. So the operand's loads are off the chain and the poll
+// that starts step C is not queued behind them (a wave's loads return in order). The two groups take the same
+// barriers in the same order; step C's last one is taken by waves 4-7 alone (a barrier waits for the waves of
+// the workgroup that have not ended).
 constexpr int LS_RING = 4;
+constexpr int LS_THREADS = 2 * NTHREADS;
 template <int FWD>
 __device__ __forceinline__ void item_lsweep(const DevMat& M, const LSweepTab& lt, int p, const double* R, long ldr,
                                             float* Y, float* D, double* X, long ldx, float* Rp, int nrhs,
@@ -430,6 +439,55 @@ __device__ __forceinline__ void item_lsweep(const DevMat& M, const LSweepTab& lt
     const bool has_near = forward ? t >= 1 : t + 1 < nt;
     const int tn = forward ? t - 1 : t + 1;
 
+    if (threadIdx.x >= NTHREADS) {  // waves 4-7: step C
+        const int wc = wv - 4;      // the wave's 16 rows of the block
+        float at[NBT][KS], rb[LS_RING][KS];
+        {
+            // C's operand: Tinv_t[s, :] (forward) or U_t[s, :] (backward), stored in fragment order (k_tile_inv)
+            typedef float f4a_t __attribute__((ext_vector_type(4)));
+            const TTAD_GLOBAL f4a_t* T =
+                reinterpret_cast<const TTAD_GLOBAL f4a_t*>(gmem(forward ? lt.tfrag[t] : lt.ufrag[t])) +
+                (long)(boff_p / SB * 4 + wc) * (LS_KS / 4) * 64 + lane;
+#pragma unroll
+            for (int j = 0; j < LS_KS / 4; j++) {
+                const f4a_t v = T[j * 64];
+#pragma unroll
+                for (int q = 0; q < 4; q++) at[j / 4][(j % 4) * 4 + q] = v[q];
+            }
+        }
+        __syncthreads();                 // the far-tile table
+        if (has_near) __syncthreads();  // step B's wait
+        __syncthreads();                 // step 2
+        __syncthreads();                 // all of r'_t published (waves 0-3 poll)
+#pragma unroll
+        for (int c = 0; c < LS_RING; c++) load_rows_blk(rb[c], Rp, tg0, bt, c);
+        f4_t acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int c = 0; c < NBT; c++) {
+#pragma unroll
+            for (int k = 0; k < KS; k++)
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(at[c][k], rb[c % LS_RING][k], acc, 0, 0, 0);
+            if (c + LS_RING < NBT) load_rows_blk(rb[c % LS_RING], Rp, tg0, bt, c + LS_RING);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int m = wc * 16 + Mfma<float>::row(lane, r), n = lane & 15;
+            if (m < w) {
+                const long gi = g0 + m;
+                if (forward) {
+                    gmem(Y)[swz(tg0, boff_p + m, n)] = acc[r];
+                } else {
+                    gmem(D)[swz(tg0, boff_p + m, n)] = acc[r];
+                    if (n < nrhs) gmem(X)[gi + (long)n * ldx] += (double)acc[r];
+                }
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        __syncthreads();  // waves 4-7 only: waves 0-3 have ended
+        if (threadIdx.x == NTHREADS) __hip_atomic_fetch_add(ocnt + t, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+
     // this block's right-hand side (MFMA result layout), off the chain
     float rhs[4];
 #pragma unroll
@@ -448,7 +506,7 @@ __device__ __forceinline__ void item_lsweep(const DevMat& M, const LSweepTab& lt
     // to LDS first, so that the stream reads no table from global memory (such a load's wait would drain it).
     const int nfar = forward ? max(0, t - 1) : max(0, nt - t - 2);
     auto far_u = [&](int f) { return forward ? f : nt - 1 - f; };
-    for (int f = threadIdx.x; f < nfar; f += NTHREADS) {
+    for (int f = threadIdx.x; f < nfar; f += NTHREADS) {  // waves 0-3
         const int u = far_u(f);
         ft[f] = FarTile{(const float*)M.tiles[forward ? t + (long)u * nt : u + (long)t * nt], M.bsz[u], M.boff[u]};
     }
@@ -559,54 +617,15 @@ __device__ __forceinline__ void item_lsweep(const DevMat& M, const LSweepTab& lt
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the stores are in the L2 before the barrier
     __syncthreads();                     // (also: every wave is done reading As)
     if (threadIdx.x == 0) __hip_atomic_fetch_add(rcnt + t, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    // C. output rows: Tinv_t[s, :] r'_t (forward) or U_t[s, :] r''_t (backward). The operand is stored in fragment
-    // order (k_tile_inv): 32 loads of 16 bytes per lane, issued before the wait for the rest of r'_t.
-    float at[NBT][KS];
-    {
-        typedef float f4a_t __attribute__((ext_vector_type(4)));
-        const TTAD_GLOBAL f4a_t* T = reinterpret_cast<const TTAD_GLOBAL f4a_t*>(gmem(forward ? lt.tfrag[t] : lt.ufrag[t])) +
-                                     (long)(boff_p / SB * 4 + wv) * (LS_KS / 4) * 64 + lane;
-#pragma unroll
-        for (int j = 0; j < LS_KS / 4; j++) {
-            const f4a_t v = T[j * 64];
-#pragma unroll
-            for (int q = 0; q < 4; q++) at[j / 4][(j % 4) * 4 + q] = v[q];
-        }
-    }
+    // all of r'_t published: waves 4-7 take step C
     ls_wait(rcnt + t, nbt, err, spin_limit, false);
-#pragma unroll
-    for (int c = 0; c < LS_RING; c++) load_rows_blk(rb[c], Rp, tg0, bt, c);
-    acc = f4_t{0, 0, 0, 0};
-#pragma unroll
-    for (int c = 0; c < NBT; c++) {
-#pragma unroll
-        for (int k = 0; k < KS; k++)
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(at[c][k], rb[c % LS_RING][k], acc, 0, 0, 0);
-        if (c + LS_RING < NBT) load_rows_blk(rb[c % LS_RING], Rp, tg0, bt, c + LS_RING);
-    }
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-        const int m = wv * 16 + Mfma<float>::row(lane, r), n = lane & 15;
-        if (m < w) {
-            const long gi = g0 + m;
-            if (forward) {
-                gmem(Y)[swz(tg0, boff_p + m, n)] = acc[r];
-            } else {
-                gmem(D)[swz(tg0, boff_p + m, n)] = acc[r];
-                if (n < nrhs) gmem(X)[gi + (long)n * ldx] += (double)acc[r];
-            }
-        }
-    }
-    __builtin_amdgcn_s_waitcnt(0x0F70);
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_fetch_add(ocnt + t, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // One workgroup per 64-row block, claimed through a ticket counter in sweep order: every block a workgroup
 // waits on (earlier tiles, and the blocks of its own tile) holds a smaller ticket or belongs to its own tile,
 // whose blocks hold consecutive tickets, so it is running or done. FWD: forward sweep (else backward).
 template <int FWD>
-__global__ __launch_bounds__(NTHREADS) void k_lsweep(const DevMat* Mp, LSweepTab tab, const double* R, long ldr,
+__global__ __launch_bounds__(LS_THREADS) void k_lsweep(const DevMat* Mp, LSweepTab tab, const double* R, long ldr,
                                                      float* Y, float* D, double* X, long ldx, float* Rp, int nrhs,
                                                      int32_t* flags, int32_t* ticket, int32_t* err,
                                                      int64_t spin_limit) {
