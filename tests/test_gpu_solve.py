@@ -266,19 +266,20 @@ def test_solve_ir_zero_rhs_and_padded_ld(ctx, oracle):
     assert backward_error(A, X[:, 1:], B[:, 1:]) <= 2e-13
 
 
-@pytest.mark.parametrize("dst", ["f32", "f64"])
-def test_matrix_copy_ragged_tiles(ctx, dst):
+@pytest.mark.parametrize("src,dst", [("f64", "f32"), ("f64", "f64"), ("f32", "f64"), ("f32", "f32")])
+def test_matrix_copy_ragged_tiles(ctx, src, dst):
     # tiles of 130, 67 and 3 rows: element counts divisible by four (copied in quads) and not (one by one);
-    # FP64 -> FP32 rounds to nearest like numpy's astype
+    # every storage pair (one kernel each); FP64 -> FP32 rounds to nearest like numpy's astype, FP32 -> FP64
+    # is exact
     rng = np.random.default_rng(3)
     sizes = [130, 67, 3]
     n = sum(sizes)
     A = rng.standard_normal((n, n))
-    M = tt.TiledMatrix(ctx, sizes, "f64")
+    M = tt.TiledMatrix(ctx, sizes, src)
     M.set(A)
     W = tt.TiledMatrix(ctx, sizes, dst)
     W.copy_from(M)
     exp = np.tril(A)
-    if dst == "f32":
+    if "f32" in (src, dst):
         exp = exp.astype(np.float32).astype(np.float64)
     assert np.array_equal(W.get(), exp)
