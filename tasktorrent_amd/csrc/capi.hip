@@ -79,7 +79,7 @@ static std::shared_ptr<StreamPlan> build_stream_plan(ttad_matrix* M, int prec, i
         t.j = j;
         t.k = k;
         t.n_items = kind == K_POTRF && potrf_rows_enabled() ? potrf_row_items(M->bsz[j])
-                                                            : std::max(1, items_of(kind, M->bsz[i], M->bsz[j]));
+                                                            : std::max(1, items_of(kind, M->bsz[i], M->bsz[j], (int)M->elem));
         t.bucket = 0;
         t.crit = 0;
         return t;
@@ -740,7 +740,7 @@ static int graph_prepare(ttad_graph* g) {
             if ((x.kind == K_GEMM || x.kind == K_SYRK) && ((x.prec == P_F64) != (M->storage == P_F64)))
                 return set_error(TTAD_ERR_INVALID, "graph: task %d precision does not match tile storage", t);
             x.n_items = x.kind == K_POTRF && potrf_rows_enabled() ? potrf_row_items(M->bsz[x.j])
-                                                                  : std::max(1, items_of(x.kind, M->bsz[x.i], M->bsz[x.j]));
+                                                                  : std::max(1, items_of(x.kind, M->bsz[x.i], M->bsz[x.j], (int)M->elem));
         }
         int needed = 0;
         for (auto& x : g->tasks)

@@ -233,13 +233,13 @@ __device__ __forceinline__ bool exec_item(const DevTask& t, int it, const DevMat
             if constexpr ((VM & V_F64S) != 0)
                 if (f64s) {
                     trsm_call<double, double, VM>((const double*)tile(j, j), n, M.dinv[j], (double*)tile(i, j), m, m, n,
-                                              it * TRSM_BM, lds, nullptr);
+                                              it * trsm_rows(8), lds, nullptr);
                     return true;
                 }
             if constexpr ((VM & V_F32S) != 0)
                 if (!f64s) {
                     trsm_call<float, float, VM>((const float*)tile(j, j), n, M.dinv[j], (float*)tile(i, j), m, m, n,
-                                            it * TRSM_BM, lds, planes(i, j));
+                                            it * trsm_rows(4), lds, planes(i, j));
                     return true;
                 }
             return false;

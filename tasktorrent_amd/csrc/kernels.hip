@@ -364,7 +364,7 @@ template <typename TS, typename TC>
 __global__ __launch_bounds__(NTHREADS) void k_raw_trsm(const TS* L, long ldl, const double* dinv, TS* X, long ldx,
                                                        int m, int n) {
     extern __shared__ __align__(16) char lds[];
-    item_trsm<TS, TC>(L, ldl, dinv, X, ldx, m, n, blockIdx.x * TRSM_BM, lds);
+    item_trsm<TS, TC>(L, ldl, dinv, X, ldx, m, n, blockIdx.x * trsm_rows(sizeof(TS)), lds);
 }
 
 // Inverses of the 64x64 diagonal blocks of a lower-triangular n x n L (one block per workgroup).
@@ -600,7 +600,8 @@ void launch_trtri_diag(int prec, const void* L, long ldl, int n, double* dinv, h
 
 void launch_raw_trsm(int prec, const void* L, long ldl, const double* dinv, void* X, long ldx, int m, int n,
                      hipStream_t s) {
-    const int items = (m + TRSM_BM - 1) / TRSM_BM;
+    const int bm = trsm_rows(prec == P_F64 ? 8 : 4);
+    const int items = (m + bm - 1) / bm;
     if (items <= 0) return;
     if (prec == P_F64)
         hipLaunchKernelGGL((k_raw_trsm<double, double>), dim3(items), dim3(NTHREADS), LDS_TRSM, s,

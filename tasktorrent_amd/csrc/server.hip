@@ -361,7 +361,8 @@ static std::shared_ptr<Template> build_template(const std::vector<int>& bsz, int
         for (size_t q = 0; q < desc.size(); q++) {
             const ttad_task_desc& d = desc[q];
             const int kind = d.kind;
-            const int items = kind == K_POTRF ? potrf_row_items(bsz[d.j]) : items_of(kind, bsz[d.i], bsz[d.j]);
+            const int items =
+                kind == K_POTRF ? potrf_row_items(bsz[d.j]) : items_of(kind, bsz[d.i], bsz[d.j], prec == P_F64 ? 8 : 4);
             const int dp = (kind == K_GEMM || kind == K_SYRK) ? prec : (prec == P_F64 ? P_F64 : P_F32);
             const int id = add(kind, items, d.i, d.j, d.k, indeg[q]);
             T->tasks[id].prec = dp;

@@ -1185,7 +1185,7 @@ __device__ __forceinline__ void item_potrf_rows(TS* __restrict__ A, long lda, in
 }
 
 // ---------------------------------------------------------------------------
-// TRSM work item: rows [r0, r0+TRSM_BM) of X (m x n) := X L^{-T} (Right, Lower,
+// TRSM work item: rows [r0, r0+trsm_rows(sizeof(TS))) of X (m x n) := X L^{-T} (Right, Lower,
 // Trans, NonUnit), in place, by 64-column blocks: the block's right-hand side goes
 // into the accumulators, the already solved columns times L's off-diagonal blocks
 // are subtracted (one GEMM), and the result is multiplied by the inverse of L's
@@ -1199,9 +1199,10 @@ template <typename TS, typename TC>
 __device__ __forceinline__ void item_trsm(const TS* __restrict__ L, long ldl, const double* __restrict__ dinv, TS* __restrict__ X,
                           long ldx, int m, int n, int r0, char* lds, void* planes = nullptr) {
     constexpr bool VEC = std::is_same<TS, TC>::value;
-    using G = GemmNT<TS, TS, TC, TRSM_BM, DB, trsm_bk<TS>(), VEC, true>;  // update from the solved columns
-    using G3 = GemmNT<TC, TC, TC, TRSM_BM, DB, DB, false, true>;  // times Dinv^T, both operands in LDS
-    const int mv = min(TRSM_BM, m - r0);
+    constexpr int BMT = trsm_rows(sizeof(TS));
+    using G = GemmNT<TS, TS, TC, BMT, DB, trsm_bk<TS>(), VEC, true>;  // update from the solved columns
+    using G3 = GemmNT<TC, TC, TC, BMT, DB, DB, false, true>;  // times Dinv^T, both operands in LDS
+    const int mv = min(BMT, m - r0);
     if (mv <= 0) return;
     TS* Xr = X + r0;
     const int nblk = (n + DB - 1) / DB;
@@ -1290,8 +1291,8 @@ constexpr int LDS_POTRF = PotrfGemm<double>::G::LDS_BYTES > 2 * DB * DP * 8 ? Po
 constexpr int LDS_POTRF_ROWS = GemmNT<double, double, double, DB, DB, DB, false, true>::STAGE * 8;
 constexpr int LDS_TRSM = cmax(GemmNT<double, double, double, TRSM_BM, DB, 32, true, true>::LDS_BYTES,
                               GemmNT<double, double, double, TRSM_BM, DB, DB, false, true>::STAGE * 8);
-constexpr int LDS_TRSM_F32 = cmax(GemmNT<float, float, float, TRSM_BM, DB, trsm_bk<float>(), true, true>::LDS_BYTES,
-                                  GemmNT<float, float, float, TRSM_BM, DB, DB, false, true>::STAGE * 4);
+constexpr int LDS_TRSM_F32 = cmax(GemmNT<float, float, float, TRSM_BM_F32, DB, trsm_bk<float>(), true, true>::LDS_BYTES,
+                                  GemmNT<float, float, float, TRSM_BM_F32, DB, DB, false, true>::STAGE * 4);
 constexpr int LDS_ITEM_MAX =
     cmax(cmax(cmax(LDS_GEMM_F64, LDS_GEMM_F32), cmax(LDS_GEMM_BF16, LDS_POTRF)), cmax(LDS_TRSM, LDS_POTRF_ROWS));
 

@@ -11,7 +11,9 @@ enum Prec : int32_t { P_F64 = 0, P_F32 = 1, P_BF16 = 2, P_BF16X3 = 3 };
 
 constexpr int NTHREADS = 256;  // one workgroup = 4 wave64
 constexpr int GEMM_BM = 128;   // GEMM/SYRK work item = 128x128 output sub-tile
-constexpr int TRSM_BM = 32;    // TRSM work item = 32-row block of the tile (its latency is on the critical path)
+constexpr int TRSM_BM = 32;      // TRSM work item of an FP64 tile: 32-row block (its latency is on the critical path)
+constexpr int TRSM_BM_F32 = 64;  // of an FP32 tile: 64-row block (half the items: fewer re-reads of the diagonal tile)
+__host__ __device__ constexpr int trsm_rows(int elem_bytes) { return elem_bytes == 8 ? TRSM_BM : TRSM_BM_F32; }
 constexpr int DB = 64;         // diagonal sub-block size whose inverse POTRF leaves for TRSM
 
 // One tiled symmetric matrix in HBM. Tile (i,j), i >= j, is a column-major
@@ -50,10 +52,10 @@ struct DevTask {
     int32_t accum; // GEMM/SYRK: add the product into C under the sub-tile lock (accumulate mode)
 };
 
-// Work items per task; host and device must agree.
-__host__ __device__ inline int items_of(int kind, int bi, int bj) {
+// Work items per task of tiles with elem_bytes-byte elements; host and device must agree.
+__host__ __device__ inline int items_of(int kind, int bi, int bj, int elem_bytes) {
     if (kind == K_POTRF) return 1;
-    if (kind == K_TRSM) return (bi + TRSM_BM - 1) / TRSM_BM;
+    if (kind == K_TRSM) return (bi + trsm_rows(elem_bytes) - 1) / trsm_rows(elem_bytes);
     const int tr = (bi + GEMM_BM - 1) / GEMM_BM;
     if (kind == K_SYRK) return tr * (tr + 1) / 2;
     if (kind == K_GEMM) return tr * ((bj + GEMM_BM - 1) / GEMM_BM);
