@@ -100,7 +100,7 @@ def cpu_oracle_run(A, b: int, threads: int, prio: int, runs: int):
 
 
 def cpu_baseline(A_c3, b: int, prio: int, c3: bool):
-    """CPU baseline beside the GPU: the oracle on the host cores at configs 1 and 2 and (c3) on this run's own
+    """CPU baseline beside the GPU: the oracle on the host cores at n = 2048 (b = 256) and n = 16384 (b = 512) and (c3) on this run's own
     n = 32768 matrix. `value` is the largest configuration run."""
     import numpy as np
     from oracle import oracle as orc
@@ -109,19 +109,19 @@ def cpu_baseline(A_c3, b: int, prio: int, c3: bool):
     threads, how = host_threads()
     log(f"cpu baseline: {threads} threads ({how}), backend {kind}")
     out = {"unit": "TFLOP/s", "cores": threads, "kind": "port", "threads_from": how, "backend": kind}
-    # config 1: n = 2048, b = 256, A = B B^T/n + I (seed 2020), median of 5
+    # small case: n = 2048, b = 256, A = B B^T/n + I (seed 2020), median of 5
     A1 = orc.spd_bbt(2048, 2020)
     t1, L1 = cpu_oracle_run(A1, 256, threads, prio, 5)
     out["c1"] = {"n": 2048, "b": 256, "s": round(t1, 5), "tflops": round(flops(2048) / t1 / 1e12, 4),
                  "factor_residual": orc.factor_residual(A1, L1), "runs": 5}
     log(f"cpu c1: {out['c1']}")
-    # config 2: n = 16384, b = 512, A = B B^T/n + I (torch seed 2021 on the GPU, copied to the host), once
+    # mid-size case: n = 16384, b = 512, A = B B^T/n + I (torch seed 2021 on the GPU, copied to the host), once
     A2 = make_matrix(16384, 2021, 0, 1.0, False).cpu().numpy()
     t2, _ = cpu_oracle_run(np.asfortranarray(A2), 512, threads, prio, 1)
     del A2
     out["c2"] = {"n": 16384, "b": 512, "s": round(t2, 3), "tflops": round(flops(16384) / t2 / 1e12, 4), "runs": 1}
     log(f"cpu c2: {out['c2']}")
-    out.update(value=out["c2"]["tflops"], sample=f"FP64 tiled Cholesky n=16384 b=512 (config 2), one run, "
+    out.update(value=out["c2"]["tflops"], sample=f"FP64 tiled Cholesky n=16384 b=512, one run, "
                                                   f"{t2:.2f} s seed->join")
     if c3 and A_c3 is not None:
         t3, L3 = cpu_oracle_run(A_c3, b, threads, prio, 1)
@@ -232,7 +232,7 @@ def _request_matrices(ctx, sizes, seeds, kappa_inner=None, dexps=None, tile=256)
 
 
 def serving_leg(ctx, rates, n_req: int, seed: int = 7, prec: str = "bf16x3", budget: int = 32 << 30):
-    """Config 4: Poisson arrivals of SPD solves, continuous batching in one resident kernel, at each rate.
+    """Serving workload: Poisson arrivals of SPD solves, continuous batching in one resident kernel, at each rate.
 
     Every request has its own matrix (n = 64*round(exp(U(ln 64, ln 4096))/64), A = B B^T/n + I, B from the
     request's seed) except ~10% that solve new right-hand sides with the kept factor of an earlier request;
@@ -286,7 +286,7 @@ def serving_leg(ctx, rates, n_req: int, seed: int = 7, prec: str = "bf16x3", bud
 
 
 def stress_leg(ctx, rate: float, n_req: int, seed: int = 11, prec: str = "bf16x3", budget: int = 200 << 30):
-    """Config 5: heavy-tailed sizes n = 256*round(min(16384, Pareto(1.5, x_m = 256))/256), kappa = 10^L with L
+    """Stress serving workload: heavy-tailed sizes n = 256*round(min(16384, Pareto(1.5, x_m = 256))/256), kappa = 10^L with L
     uniform in [2, 10] through A = D (B B^T/n + delta I) D: delta chosen per request so that the inner matrix
     has condition number exactly kappa (B n x n/2), D_ii = 10^(U_i / 2) (the scaling of the formula, at most a
     factor 10 more on cond(A); Cholesky and the refinement are nearly invariant to it). 1-8 RHS, admission by
@@ -454,12 +454,12 @@ def main():
     ap.add_argument("--prec", default="bf16x3", help="headline mixed precision")
     ap.add_argument("--extra-precs", default="f32,bf16", help="other mixed legs (one timed run each)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-cpu-c3", action="store_true", help="CPU baseline at configs 1 and 2 only")
+    ap.add_argument("--no-cpu-c3", action="store_true", help="CPU baseline at n = 2048 and n = 16384 only")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--serve-rates", default="200,800,1600,3200,6400", help="Poisson request rates of the serving leg (1/s)")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--stress-rate", type=float, default=50.0, help="request rate of the config-5 stress leg (1/s)")
+    ap.add_argument("--stress-rate", type=float, default=50.0, help="request rate of the stress serving leg (1/s)")
     ap.add_argument("--stress-requests", type=int, default=400)
     ap.add_argument("--no-stress", action="store_true")
     ap.add_argument("--no-update", action="store_true", help="skip the FP64 trailing-update and panel-step legs")

@@ -222,7 +222,7 @@ def host_threads():
 
 
 def test_c2_fp64_n16384_matches_oracle(ctx, oracle):
-    # config 2: FP64 tiled Cholesky through the device task DAG at n = 16384, tile 512, A = B B^T/n + I from
+    # mid-size case: FP64 tiled Cholesky through the device task DAG at n = 16384, tile 512, A = B B^T/n + I from
     # seed 2021; L against the oracle's Taskflow + sequential-BLAS factor (threads = host cores) within 1e-12
     # relative, and the device's own ||L L^T - A||_F / ||A||_F
     n, b = 16384, 512
@@ -240,7 +240,7 @@ def test_c2_fp64_n16384_matches_oracle(ctx, oracle):
     W.close()
     M.close()
     err = rel(L, Lref)
-    print(f"config 2: n={n} b={b} rel(L, L_oracle)={err:.3e} ||LL^T-A||/||A||={res:.3e} "
+    print(f"mid-size case: n={n} b={b} rel(L, L_oracle)={err:.3e} ||LL^T-A||/||A||={res:.3e} "
           f"oracle {st['total']:.2f} s on {host_threads()} threads", flush=True)
     assert err < 1e-12
     assert res < 1e-14
