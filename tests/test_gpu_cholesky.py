@@ -221,7 +221,7 @@ def host_threads():
     return max(1, min(n, int(env))) if env.isdigit() and int(env) > 0 else n
 
 
-def test_c2_fp64_n16384_matches_oracle(ctx, oracle):
+def test_fp64_n16384_b512_matches_oracle(ctx, oracle):
     # mid-size case: FP64 tiled Cholesky through the device task DAG at n = 16384, tile 512, A = B B^T/n + I from
     # seed 2021; L against the oracle's Taskflow + sequential-BLAS factor (threads = host cores) within 1e-12
     # relative, and the device's own ||L L^T - A||_F / ||A||_F
